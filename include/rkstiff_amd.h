@@ -1,0 +1,149 @@
+/* rkstiff_amd: MI355X (gfx950) stepping engine for rkstiff's diagonal solvers.
+ *
+ * C ABI of librkstiff_amd.so.  Plain pointers and sizes; state arrays are
+ * device pointers (complex interleaved, float64x2 for RKS_F64, float32x2 for
+ * RKS_F32); per-instance scalars are host arrays.  Functions return RKS_OK or an
+ * error code; rks_last_error() gives the message (thread-local).
+ *
+ * The reference (rkstiff, pure Python/NumPy) has no FFI: each entry point below
+ * replaces one Python-level operation of the reference, cited per function.
+ * A plan owns its workspace and device records, is bound to one device and is
+ * not thread-safe (like the reference's solver object); distinct plans may run
+ * concurrently on distinct streams.
+ */
+#ifndef RKSTIFF_AMD_H
+#define RKSTIFF_AMD_H
+
+#include <hip/hip_runtime_api.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status / error codes */
+enum {
+  RKS_OK = 0,
+  RKS_ERR_INVALID = 1, /* ValueError in the reference's terms */
+  RKS_ERR_HIP = 2,     /* runtime failure */
+  RKS_MAXLOOPS = 3,    /* BaseSolverAS.MaxLoopsExceeded (solveras.py:399-406) */
+  RKS_MINSTEP = 4,     /* BaseSolverAS.MinimumStepReached (solveras.py:407-410) */
+  RKS_PAUSED = 5,      /* evolve paused: snapshot buffer full, call rks_resume */
+  RKS_RUNNING = 6
+};
+
+/* methods: the reference's seven diagonal solver classes */
+enum {
+  RKS_ETD35 = 0, /* etd35.py:723-889 */
+  RKS_ETD34 = 1, /* etd34.py:439-621 */
+  RKS_ETD4 = 2,  /* etd4.py:311-422 */
+  RKS_ETD5 = 3,  /* etd5.py:485-655 */
+  RKS_IF34 = 4,  /* if34.py:260-338 */
+  RKS_IF45DP = 5,/* if45dp.py:8-249 */
+  RKS_IF4 = 6    /* if4.py:202-275 */
+};
+
+/* built-in nonlinear terms (the user nl_func patterns of the reference) */
+enum {
+  RKS_NL_ADVECT_RFFT = 0, /* c * rfft(irfft(u) * irfft(i kx u)): KS/Burgers (c=-1, README.md:94-97,
+                             models.py:189-192), KdV (c=-6, models.py:140-143); n real points */
+  RKS_NL_CUBIC_FFT = 1,   /* i g fft(|v|^2 v), v = ifft(u): cubic NLS (demos/nls.ipynb cell 29); n points */
+  RKS_NL_NONE = 2         /* caller evaluates nl (generic path): only coefficients, stage combinations
+                             and the error norm run here; n = number of modes */
+};
+
+enum { RKS_F64 = 0, RKS_F32 = 1 };
+
+/* SolverConfig (solveras.py:27-169), the class constants MAX_LOOPS/MAX_S/MIN_S
+ * (solveras.py:255-257) and ETDConfig (etd.py:46-131), passed by value. */
+typedef struct rks_config {
+  double epsilon, incr_f, decr_f, safety_f, adapt_cutoff, minh;
+  double max_s, min_s;
+  int max_loops;
+  int contour_points;
+  double modecutoff, contour_radius;
+} rks_config;
+
+/* One trajectory's operator (solver.py:91-136: lin_op 1-D => diagonal). */
+typedef struct rks_instance_desc {
+  int n;               /* real-space points (power of two, 16..8192) or modes for RKS_NL_NONE */
+  int L_is_complex;    /* lin_op dtype: 0 float64[m], 1 complex128[m] */
+  const void* L;       /* lin_op, float64 data, host or device pointer (copied) */
+  const double* kx;    /* wavenumbers float64[m] (RKS_NL_ADVECT_RFFT), host or device (copied) */
+  double nl_coef;      /* c of the advective term / g of the cubic term */
+} rks_instance_desc;
+
+typedef struct rks_stats {
+  double tc;           /* current time (evolve) */
+  double h_used;       /* last accepted step */
+  double h_next;       /* suggested next step (step(): third return value) */
+  double h;            /* next step the evolve loop would try */
+  int steps, attempts, rejects, status;
+  int log_count, snap_count, nl_calls, coef_updates;
+} rks_stats;
+
+typedef struct rks_plan rks_plan;
+
+/* Create a plan for `ninst` trajectories of one method / nonlinearity / precision.
+ * Replaces the solver constructors (e.g. ETD35.__init__, etd35.py:784-830, and
+ * BaseSolverAS.__init__, solveras.py:262-282). */
+int rks_plan_create(int method, int nl_kind, int precision, const rks_config* cfg, int ninst,
+                    const rks_instance_desc* desc, rks_plan** out);
+int rks_plan_destroy(rks_plan* p);
+int rks_plan_info(const rks_plan* p, int* ninst, int* nslot, int* ncoef, int* esize);
+/* Replace the controller/ETD configuration (the reference reads solver.config and the
+ * MAX_LOOPS/MAX_S/MIN_S attributes at every step, solveras.py:384-410). */
+int rks_set_config(rks_plan* p, const rks_config* cfg, hipStream_t stream);
+
+/* Snapshot storage (store_data / store_freq of evolve, solveras.py:609-611,643-646)
+ * and the per-attempt (h, s) log.  snap_cap snapshots per instance; an evolve
+ * pauses with RKS_PAUSED when the buffer is full. */
+int rks_set_storage(rks_plan* p, int store_freq, int snap_cap, int log_cap);
+
+/* Integrate every instance from t0[i] to tf[i]: BaseSolverAS.evolve
+ * (solveras.py:556-650; h[i] = h_init, clamped to tf-t0 like 620-621) or
+ * BaseSolverCS.evolve (solvercs.py:191-279; h[i] = constant step).  u0[i] is
+ * read, uf[i] receives u(tf).  Asynchronous on `stream`. */
+int rks_evolve(rks_plan* p, const void* const* u0, void* const* uf, const double* t0, const double* tf,
+               const double* h, hipStream_t stream);
+/* Continue trajectories paused on a full snapshot buffer. */
+int rks_resume(rks_plan* p, hipStream_t stream);
+
+/* One accepted step from u[i] with suggested h[i]: BaseSolverAS.step
+ * (solveras.py:336-410) or BaseSolverCS.step (solvercs.py:163-186).  Solver state
+ * (FSAL NL1, coefficient cache, accept flag) persists between calls. */
+int rks_step(rks_plan* p, const void* const* u, void* const* u_new, const double* h, hipStream_t stream);
+/* BaseSolverAS.reset / BaseSolverCS.reset: clear FSAL and coefficient state. */
+int rks_reset(rks_plan* p, hipStream_t stream);
+
+/* Evaluate the built-in nonlinear term: nl_func(u) (models.py:140-143, README.md:94-97). */
+int rks_nl_eval(rks_plan* p, const void* const* u, void* const* nl_out, hipStream_t stream);
+
+/* Coefficient arrays of instance `inst` at step h (e.g. _Etd35Diagonal.update_coeffs,
+ * etd35.py:157-288), computed in FP64 and stored in the plan precision.
+ * dst (optional, device) receives ncoef x m complex values. */
+int rks_coeffs(rks_plan* p, int inst, double h, void* dst, hipStream_t stream);
+/* Generic-nl path: one stage combination k = E u + sum C NL (e.g. etd35.py:320-343),
+ * stage = -1 gives the embedded error estimate (etd35.py:344).  slots: nslot device
+ * pointers to the NL arrays.  Coefficients come from the last rks_coeffs call. */
+int rks_combine(rks_plan* p, int inst, int stage, double h, const void* u, const void* const* slots, void* out,
+                hipStream_t stream);
+/* Generic-nl path: s of BaseSolverAS._compute_s (solveras.py:412-455) into s_dev (device double). */
+int rks_error_s(rks_plan* p, int inst, const void* u_new, const void* err, double* s_dev, hipStream_t stream);
+
+/* Synchronise `stream` and read per-instance counters/status (host array of ninst). */
+int rks_get_stats(rks_plan* p, rks_stats* out, hipStream_t stream);
+/* Attempt log of instance `inst`: cap (h, s) pairs into host_hs. */
+int rks_read_log(rks_plan* p, int inst, double* host_hs, int cap, hipStream_t stream);
+/* Copy `count` stored snapshots of `inst` to dst_dev (device, count x m complex) and
+ * their times to t_host; marks the buffer consumed. */
+int rks_read_snapshots(rks_plan* p, int inst, void* dst_dev, double* t_host, int count, hipStream_t stream);
+
+/* Method tableau (host-only, no device needed): fills `bytes` >= rks_sizeof(0). */
+int rks_get_tableau(int method, void* out, int bytes);
+int rks_sizeof(int what);
+const char* rks_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* RKSTIFF_AMD_H */

@@ -1,0 +1,159 @@
+"""ctypes binding of librkstiff_amd.so (the C ABI declared in include/rkstiff_amd.h).
+
+The library is the only compute path of this package: importing a solver that
+needs it raises ``RuntimeError`` when the shared object is missing, and every
+call that launches device work requires a HIP device.  There is no CPU fallback.
+"""
+import ctypes
+import os
+import threading
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "librkstiff_amd.so")
+
+RKS_OK, RKS_ERR_INVALID, RKS_ERR_HIP, RKS_MAXLOOPS, RKS_MINSTEP, RKS_PAUSED, RKS_RUNNING = range(7)
+METHOD_IDS = {"ETD35": 0, "ETD34": 1, "ETD4": 2, "ETD5": 3, "IF34": 4, "IF45DP": 5, "IF4": 6}
+NL_ADVECT_RFFT, NL_CUBIC_FFT, NL_NONE = 0, 1, 2
+F64, F32 = 0, 1
+
+EXPORTS = (
+    "rks_plan_create", "rks_plan_destroy", "rks_plan_info", "rks_set_storage", "rks_evolve",
+    "rks_resume", "rks_step", "rks_reset", "rks_nl_eval", "rks_coeffs", "rks_combine",
+    "rks_error_s", "rks_get_stats", "rks_read_log", "rks_read_snapshots", "rks_get_tableau",
+    "rks_sizeof", "rks_last_error",
+)
+
+
+class Config(ctypes.Structure):
+    _fields_ = [("epsilon", ctypes.c_double), ("incr_f", ctypes.c_double),
+                ("decr_f", ctypes.c_double), ("safety_f", ctypes.c_double),
+                ("adapt_cutoff", ctypes.c_double), ("minh", ctypes.c_double),
+                ("max_s", ctypes.c_double), ("min_s", ctypes.c_double),
+                ("max_loops", ctypes.c_int), ("contour_points", ctypes.c_int),
+                ("modecutoff", ctypes.c_double), ("contour_radius", ctypes.c_double)]
+
+
+class InstanceDesc(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int), ("L_is_complex", ctypes.c_int),
+                ("L", ctypes.c_void_p), ("kx", ctypes.c_void_p), ("nl_coef", ctypes.c_double)]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [("tc", ctypes.c_double), ("h_used", ctypes.c_double),
+                ("h_next", ctypes.c_double), ("h", ctypes.c_double),
+                ("steps", ctypes.c_int), ("attempts", ctypes.c_int),
+                ("rejects", ctypes.c_int), ("status", ctypes.c_int),
+                ("log_count", ctypes.c_int), ("snap_count", ctypes.c_int),
+                ("nl_calls", ctypes.c_int), ("coef_updates", ctypes.c_int)]
+
+
+# tableau mirror (rkstiff_amd/csrc/rks_types.h)
+MAX_STAGES, MAX_TERMS, MAX_COEF, MAX_BASIS, MAX_SLOTS = 8, 8, 32, 16, 8
+
+
+class Term(ctypes.Structure):
+    _fields_ = [("src", ctypes.c_int), ("src2", ctypes.c_int), ("coef", ctypes.c_int),
+                ("hmode", ctypes.c_int), ("scale", ctypes.c_double), ("scale2", ctypes.c_double),
+                ("den", ctypes.c_double)]
+
+
+class Stage(ctypes.Structure):
+    _fields_ = [("ecoef", ctypes.c_int), ("nterm", ctypes.c_int), ("has_nl", ctypes.c_int),
+                ("nl_slot", ctypes.c_int), ("hsum", ctypes.c_int), ("pad", ctypes.c_int),
+                ("t", Term * MAX_TERMS)]
+
+
+class Basis(ctypes.Structure):
+    _fields_ = [("r", ctypes.c_int), ("fnum", ctypes.c_int), ("fden", ctypes.c_int),
+                ("pad", ctypes.c_int)]
+
+
+class Coef(ctypes.Structure):
+    _fields_ = [("nb", ctypes.c_int), ("hmul", ctypes.c_int), ("basis", ctypes.c_int * 4),
+                ("w", ctypes.c_double * 4), ("outer", ctypes.c_double), ("den", ctypes.c_double)]
+
+
+class Tableau(ctypes.Structure):
+    _fields_ = [("method", ctypes.c_int), ("adaptive", ctypes.c_int), ("q", ctypes.c_int),
+                ("nl1_policy", ctypes.c_int), ("fsal_slot", ctypes.c_int), ("nslot", ctypes.c_int),
+                ("nstage", ctypes.c_int), ("nerr", ctypes.c_int), ("ncoef", ctypes.c_int),
+                ("nbasis", ctypes.c_int), ("uses_contour", ctypes.c_int), ("err_coef", ctypes.c_int),
+                ("err_factored", ctypes.c_int), ("err_hpow", ctypes.c_int), ("pad2", ctypes.c_int),
+                ("err_scale", ctypes.c_double),
+                ("st", Stage * MAX_STAGES), ("err", Term * MAX_TERMS),
+                ("coef", Coef * MAX_COEF), ("basis", Basis * MAX_BASIS)]
+
+
+_lib = None
+_lock = threading.Lock()
+
+
+def load(path=None):
+    """Load the shared library once; raise RuntimeError if it is not built."""
+    global _lib
+    with _lock:
+        if _lib is not None:
+            return _lib
+        p = path or LIB_PATH
+        if not os.path.exists(p):
+            raise RuntimeError(
+                f"rkstiff_amd: HIP library {p} is not built; run __graft_entry__.build() "
+                "(hipcc, gfx950). There is no CPU fallback.")
+        lib = ctypes.CDLL(p)
+        vp, ip, dp = ctypes.c_void_p, ctypes.c_int, ctypes.c_double
+        P = ctypes.POINTER
+        lib.rks_plan_create.argtypes = [ip, ip, ip, P(Config), ip, P(InstanceDesc), P(vp)]
+        lib.rks_plan_destroy.argtypes = [vp]
+        lib.rks_plan_info.argtypes = [vp, P(ip), P(ip), P(ip), P(ip)]
+        lib.rks_set_storage.argtypes = [vp, ip, ip, ip]
+        lib.rks_evolve.argtypes = [vp, P(vp), P(vp), P(dp), P(dp), P(dp), vp]
+        lib.rks_resume.argtypes = [vp, vp]
+        lib.rks_step.argtypes = [vp, P(vp), P(vp), P(dp), vp]
+        lib.rks_reset.argtypes = [vp, vp]
+        lib.rks_nl_eval.argtypes = [vp, P(vp), P(vp), vp]
+        lib.rks_coeffs.argtypes = [vp, ip, dp, vp, vp]
+        lib.rks_combine.argtypes = [vp, ip, ip, dp, vp, P(vp), vp, vp]
+        lib.rks_error_s.argtypes = [vp, ip, vp, vp, vp, vp]
+        lib.rks_get_stats.argtypes = [vp, P(Stats), vp]
+        lib.rks_read_log.argtypes = [vp, ip, P(dp), ip, vp]
+        lib.rks_read_snapshots.argtypes = [vp, ip, vp, P(dp), ip, vp]
+        lib.rks_get_tableau.argtypes = [ip, vp, ip]
+        lib.rks_sizeof.argtypes = [ip]
+        lib.rks_last_error.restype = ctypes.c_char_p
+        for name in EXPORTS:
+            if name not in ("rks_last_error",):
+                getattr(lib, name).restype = ctypes.c_int
+        assert lib.rks_sizeof(0) == ctypes.sizeof(Tableau), "tableau layout mismatch"
+        _lib = lib
+        return lib
+
+
+def check(rc):
+    """Raise for a non-OK return code (ValueError for invalid arguments)."""
+    if rc == RKS_OK:
+        return
+    msg = load().rks_last_error().decode()
+    if rc == RKS_ERR_INVALID:
+        raise ValueError(msg)
+    raise RuntimeError(f"rkstiff_amd error {rc}: {msg}")
+
+
+def tableau(method):
+    """Host-side copy of a method tableau (no device needed)."""
+    t = Tableau()
+    check(load().rks_get_tableau(METHOD_IDS[method], ctypes.byref(t), ctypes.sizeof(t)))
+    return t
+
+
+def ptr_array(ptrs):
+    arr = (ctypes.c_void_p * len(ptrs))()
+    for i, p in enumerate(ptrs):
+        arr[i] = p
+    return arr
+
+
+def dbl_array(vals):
+    arr = (ctypes.c_double * len(vals))()
+    for i, v in enumerate(vals):
+        arr[i] = float(v)
+    return arr

@@ -1,0 +1,57 @@
+"""Build librkstiff_amd.so for gfx950 with hipcc (in-tree, no JIT cache).
+
+Translation units (rkstiff_amd/csrc): one per precision x nonlinearity for the
+persistent stepping kernels, the host/C-ABI unit and the tableau unit.  They
+compile in parallel and link into rkstiff_amd/librkstiff_amd.so.
+"""
+import concurrent.futures as cf
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+SRC = os.path.join(HERE, "csrc")
+OUT = os.path.join(HERE, "librkstiff_amd.so")
+BUILD = os.path.join(HERE, "_build")
+ARCH = os.environ.get("RKS_OFFLOAD_ARCH", "gfx950")
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+FLAGS = ["-O3", "-std=c++17", f"--offload-arch={ARCH}", "-fPIC",
+         "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+         "-I" + os.path.join(HERE, "..", "include"), "-Wno-unused-result"]
+UNITS = ["rks_k_f64_advect.hip", "rks_k_f64_cubic.hip", "rks_k_f32_advect.hip",
+         "rks_k_f32_cubic.hip", "rks_host.hip", "rks_tableau.cpp"]
+
+
+def _compile(unit, verbose):
+    src = os.path.join(SRC, unit)
+    obj = os.path.join(BUILD, unit + ".o")
+    lang = ["-x", "hip"] if unit.endswith(".hip") else []
+    cmd = [HIPCC] + FLAGS + lang + ["-c", src, "-o", obj]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    out = "\n".join(line for line in (r.stdout + r.stderr).splitlines()
+                     if "unknown target feature" not in line and "packed-fp32-ops" not in line)
+    if r.returncode != 0:
+        raise RuntimeError(f"hipcc failed for {unit}:\n{out}")
+    if verbose and out.strip():
+        print(out)
+    return obj
+
+
+def build(verbose=False, jobs=None):
+    """Compile every unit (always) and link the shared library; returns its path."""
+    os.makedirs(BUILD, exist_ok=True)
+    jobs = jobs or min(len(UNITS), max(1, os.cpu_count() or 1))
+    with cf.ThreadPoolExecutor(jobs) as ex:
+        objs = list(ex.map(lambda u: _compile(u, verbose), UNITS))
+    tmp = OUT + ".tmp"
+    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + objs
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("link failed:\n" + r.stdout + r.stderr)
+    os.replace(tmp, OUT)
+    return OUT
+
+
+if __name__ == "__main__":
+    t = build(verbose="-v" in sys.argv)
+    print(t)

@@ -1,0 +1,731 @@
+// Host side of the stepping engine: plans, launches and the C ABI of
+// include/rkstiff_amd.h.  Kernel instantiations live in rks_k_*.hip (one
+// translation unit per precision x nonlinearity, compiled in parallel).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "rks_kernels.h"
+
+// ---------------------------------------------------------------------------
+// generic-nl path kernels (arbitrary caller-supplied nonlinear term)
+// ---------------------------------------------------------------------------
+template <typename R>
+__global__ void rks_coef_kernel(const rks_tableau* tab, const rks_ctrl* ctl, const double* L, int Lc, int M,
+                                double h, cx<R>* coefs) {
+  for (int m = blockIdx.x * blockDim.x + threadIdx.x; m < M; m += gridDim.x * blockDim.x)
+    coef_mode<R>(*tab, *ctl, L, Lc, m, M, h, coefs);
+}
+
+struct rks_combine_args {
+  const rks_tableau* tab;
+  const void* u;
+  const void* slots[RKS_MAX_SLOTS];
+  const void* coefs;
+  void* out;
+  int stage;
+  int is_err;
+  int M;
+  int pad;
+  double h;
+};
+
+template <typename R>
+__global__ void rks_combine_kernel(rks_combine_args A) {
+  using C = cx<R>;
+  const C* u = (const C*)A.u;
+  const C* cf = (const C*)A.coefs;
+  C* out = (C*)A.out;
+  auto slotp = [&](int s) -> const C* { return (const C*)A.slots[s]; };
+  for (int m = blockIdx.x * blockDim.x + threadIdx.x; m < A.M; m += gridDim.x * blockDim.x) {
+    if (A.is_err) out[m] = error_value<R>(*A.tab, slotp, cf, (size_t)A.M, m, A.h);
+    else out[m] = stage_value<R>(A.tab->st[A.stage], u, slotp, cf, (size_t)A.M, m, A.h);
+  }
+}
+
+// s = safety * (eps ||u[idx]|| / ||err[idx]||)^(1/q), idx = |u|/max|u| > cutoff; one workgroup
+template <typename R>
+__global__ __launch_bounds__(1024) void rks_norm_kernel(const cx<R>* u, const cx<R>* err, int M, double eps,
+                                                        double safety, double cutoff, int q, double* s_out) {
+  __shared__ double red[2 * 17];
+  double mx = 0.0;
+  bool any = false;
+  for (int m = threadIdx.x; m < M; m += 1024) {
+    const double a = hypot((double)u[m].re, (double)u[m].im);
+    mx = any ? nanmax(mx, a) : a;
+    any = true;
+  }
+  mx = wg_max<1024>(mx, red);
+  double su = 0.0, se = 0.0;
+  for (int m = threadIdx.x; m < M; m += 1024) {
+    const double ur = u[m].re, ui = u[m].im, er = err[m].re, ei = err[m].im;
+    if (hypot(ur, ui) / mx > cutoff) {
+      su += ur * ur + ui * ui;
+      se += er * er + ei * ei;
+    }
+  }
+  wg_sum2<1024>(su, se, red);
+  if (threadIdx.x == 0) *s_out = safety * pow(eps * sqrt(su) / sqrt(se), 1.0 / (double)q);
+}
+
+__global__ void rks_zero_int(int* p, int n) {
+  for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0;
+}
+
+template <typename R>
+__global__ void rks_twiddle_kernel(cx<R>* tw, int N) {
+  for (int x = blockIdx.x * blockDim.x + threadIdx.x; x < N; x += gridDim.x * blockDim.x) {
+    double s, c;
+    sincospi(-2.0 * (double)x / (double)N, &s, &c);
+    tw[x] = mkc((R)c, (R)s);
+  }
+}
+
+__global__ void rks_f64_to_f32(const double* a, float* b, int n) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) b[i] = (float)a[i];
+}
+
+
+// explicit instantiations are in rks_k_<prec>_<nl>.hip
+extern template __global__ void rks_kernel<double, 4, NLK_ADVECT>(rks_launch);
+extern template __global__ void rks_kernel<double, 5, NLK_ADVECT>(rks_launch);
+extern template __global__ void rks_kernel<double, 6, NLK_ADVECT>(rks_launch);
+extern template __global__ void rks_kernel<double, 7, NLK_ADVECT>(rks_launch);
+extern template __global__ void rks_kernel<double, 8, NLK_ADVECT>(rks_launch);
+extern template __global__ void rks_kernel<double, 9, NLK_ADVECT>(rks_launch);
+extern template __global__ void rks_kernel<double, 10, NLK_ADVECT>(rks_launch);
+extern template __global__ void rks_kernel<double, 11, NLK_ADVECT>(rks_launch);
+extern template __global__ void rks_kernel<double, 12, NLK_ADVECT>(rks_launch);
+extern template __global__ void rks_kernel<double, 13, NLK_ADVECT>(rks_launch);
+extern template __global__ void rks_kernel<double, 4, NLK_CUBIC>(rks_launch);
+extern template __global__ void rks_kernel<double, 5, NLK_CUBIC>(rks_launch);
+extern template __global__ void rks_kernel<double, 6, NLK_CUBIC>(rks_launch);
+extern template __global__ void rks_kernel<double, 7, NLK_CUBIC>(rks_launch);
+extern template __global__ void rks_kernel<double, 8, NLK_CUBIC>(rks_launch);
+extern template __global__ void rks_kernel<double, 9, NLK_CUBIC>(rks_launch);
+extern template __global__ void rks_kernel<double, 10, NLK_CUBIC>(rks_launch);
+extern template __global__ void rks_kernel<double, 11, NLK_CUBIC>(rks_launch);
+extern template __global__ void rks_kernel<double, 12, NLK_CUBIC>(rks_launch);
+extern template __global__ void rks_kernel<double, 13, NLK_CUBIC>(rks_launch);
+extern template __global__ void rks_kernel<float, 4, NLK_ADVECT>(rks_launch);
+extern template __global__ void rks_kernel<float, 5, NLK_ADVECT>(rks_launch);
+extern template __global__ void rks_kernel<float, 6, NLK_ADVECT>(rks_launch);
+extern template __global__ void rks_kernel<float, 7, NLK_ADVECT>(rks_launch);
+extern template __global__ void rks_kernel<float, 8, NLK_ADVECT>(rks_launch);
+extern template __global__ void rks_kernel<float, 9, NLK_ADVECT>(rks_launch);
+extern template __global__ void rks_kernel<float, 10, NLK_ADVECT>(rks_launch);
+extern template __global__ void rks_kernel<float, 11, NLK_ADVECT>(rks_launch);
+extern template __global__ void rks_kernel<float, 12, NLK_ADVECT>(rks_launch);
+extern template __global__ void rks_kernel<float, 13, NLK_ADVECT>(rks_launch);
+extern template __global__ void rks_kernel<float, 4, NLK_CUBIC>(rks_launch);
+extern template __global__ void rks_kernel<float, 5, NLK_CUBIC>(rks_launch);
+extern template __global__ void rks_kernel<float, 6, NLK_CUBIC>(rks_launch);
+extern template __global__ void rks_kernel<float, 7, NLK_CUBIC>(rks_launch);
+extern template __global__ void rks_kernel<float, 8, NLK_CUBIC>(rks_launch);
+extern template __global__ void rks_kernel<float, 9, NLK_CUBIC>(rks_launch);
+extern template __global__ void rks_kernel<float, 10, NLK_CUBIC>(rks_launch);
+extern template __global__ void rks_kernel<float, 11, NLK_CUBIC>(rks_launch);
+extern template __global__ void rks_kernel<float, 12, NLK_CUBIC>(rks_launch);
+extern template __global__ void rks_kernel<float, 13, NLK_CUBIC>(rks_launch);
+// ===========================================================================
+// host side: plans and the C ABI
+// ===========================================================================
+#include "../../include/rkstiff_amd.h"
+
+int rks_build_tableau(int method, rks_tableau* out);
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIPCHK(x)                                                                          \
+  do {                                                                                     \
+    hipError_t e_ = (x);                                                                   \
+    if (e_ != hipSuccess) return fail(RKS_ERR_HIP, std::string(#x) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+typedef void (*kernel_fn)(rks_launch);
+
+struct KernelEntry {
+  const void* fn;
+  int threads;
+  int lds;
+};
+
+template <typename R, int LOG2N, int NLK>
+KernelEntry entry_of() {
+  KernelEntry e;
+  e.fn = (const void*)&rks_kernel<R, LOG2N, NLK>;
+  e.threads = rks_threads(LOG2N, NLK);
+  e.lds = (1 << LOG2N) * (int)sizeof(cx<R>);
+  return e;
+}
+
+template <typename R, int NLK>
+bool lookup_nlk(int log2n, KernelEntry* e) {
+  switch (log2n) {
+    case 4: *e = entry_of<R, 4, NLK>(); return true;
+    case 5: *e = entry_of<R, 5, NLK>(); return true;
+    case 6: *e = entry_of<R, 6, NLK>(); return true;
+    case 7: *e = entry_of<R, 7, NLK>(); return true;
+    case 8: *e = entry_of<R, 8, NLK>(); return true;
+    case 9: *e = entry_of<R, 9, NLK>(); return true;
+    case 10: *e = entry_of<R, 10, NLK>(); return true;
+    case 11: *e = entry_of<R, 11, NLK>(); return true;
+    case 12: *e = entry_of<R, 12, NLK>(); return true;
+    case 13: *e = entry_of<R, 13, NLK>(); return true;
+    default: return false;
+  }
+}
+
+bool lookup(int prec, int nlk, int log2n, KernelEntry* e) {
+  if (prec == RKS_F64) return nlk == NLK_ADVECT ? lookup_nlk<double, NLK_ADVECT>(log2n, e)
+                                                : lookup_nlk<double, NLK_CUBIC>(log2n, e);
+  return nlk == NLK_ADVECT ? lookup_nlk<float, NLK_ADVECT>(log2n, e) : lookup_nlk<float, NLK_CUBIC>(log2n, e);
+}
+
+int ilog2(int n) {
+  int l = 0;
+  while ((1 << l) < n) ++l;
+  return (1 << l) == n ? l : -1;
+}
+
+}  // namespace
+
+struct rks_plan {
+  int method = 0, nlk = 0, prec = 0, ninst = 0, device = 0, esize = 16, cu = 256;
+  rks_tableau tab;
+  rks_ctrl ctl;
+  std::vector<rks_inst> h_inst;
+  std::vector<rks_io> h_io;
+  std::vector<int> modes;
+  rks_tableau* d_tab = nullptr;
+  rks_ctrl* d_ctl = nullptr;
+  rks_inst* d_inst = nullptr;
+  rks_io* d_io = nullptr;
+  void* d_ws = nullptr;
+  double* d_L = nullptr;
+  void* d_kx = nullptr;
+  double* d_log = nullptr;
+  void* d_snap = nullptr;
+  double* d_snap_t = nullptr;
+  int* d_queue = nullptr;
+  int* d_order = nullptr;
+  void* d_tw[14] = {nullptr};
+  std::vector<size_t> ws_off;
+  // size buckets: instances grouped by log2(n)
+  std::vector<int> bucket_log2, bucket_start, bucket_count;
+  std::vector<int> order;
+  int snap_cap = 0, store_freq = 0, log_cap = 0;
+};
+
+static void fill_ctrl(rks_ctrl* c, const rks_config* cfg);
+
+extern "C" {
+
+const char* rks_last_error(void) { return g_err.c_str(); }
+
+int rks_get_tableau(int method, void* out, int bytes) {
+  if (bytes < (int)sizeof(rks_tableau)) return fail(RKS_ERR_INVALID, "tableau buffer too small");
+  rks_tableau t;
+  if (rks_build_tableau(method, &t)) return fail(RKS_ERR_INVALID, "unknown method");
+  std::memcpy(out, &t, sizeof(t));
+  return RKS_OK;
+}
+
+int rks_sizeof(int what) {
+  switch (what) {
+    case 0: return (int)sizeof(rks_tableau);
+    case 1: return (int)sizeof(rks_inst);
+    case 2: return (int)sizeof(rks_io);
+    case 3: return (int)sizeof(rks_ctrl);
+    default: return -1;
+  }
+}
+
+int rks_plan_create(int method, int nl_kind, int precision, const rks_config* cfg, int ninst,
+                    const rks_instance_desc* desc, rks_plan** out) {
+  if (!out || !cfg || !desc || ninst <= 0) return fail(RKS_ERR_INVALID, "null argument or ninst <= 0");
+  *out = nullptr;
+  if (precision != RKS_F64 && precision != RKS_F32) return fail(RKS_ERR_INVALID, "precision must be F64 or F32");
+  if (nl_kind != RKS_NL_ADVECT_RFFT && nl_kind != RKS_NL_CUBIC_FFT && nl_kind != RKS_NL_NONE)
+    return fail(RKS_ERR_INVALID, "unknown nonlinear kind");
+  rks_plan* p = new rks_plan();
+  if (rks_build_tableau(method, &p->tab)) {
+    delete p;
+    return fail(RKS_ERR_INVALID, "unknown method");
+  }
+  p->method = method;
+  p->nlk = nl_kind;
+  p->prec = precision;
+  p->ninst = ninst;
+  p->esize = precision == RKS_F64 ? 16 : 8;
+  fill_ctrl(&p->ctl, cfg);
+  if (cfg->contour_points < 2) {
+    delete p;
+    return fail(RKS_ERR_INVALID, "contour_points must be > 1");
+  }
+  (void)hipGetDevice(&p->device);
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, p->device) == hipSuccess) p->cu = prop.multiProcessorCount;
+  // geometry
+  size_t ws_total = 0, L_total = 0, kx_total = 0;
+  p->modes.resize(ninst);
+  p->ws_off.resize(ninst);
+  std::vector<size_t> Loff(ninst), kxoff(ninst);
+  for (int i = 0; i < ninst; ++i) {
+    const int n = desc[i].n;
+    int m;
+    if (nl_kind == RKS_NL_ADVECT_RFFT) {
+      const int l2 = ilog2(n);
+      if (l2 < 4 || l2 > 13) {
+        rks_plan_destroy(p);
+        return fail(RKS_ERR_INVALID, "rfft grid size must be a power of two in [16, 8192]");
+      }
+      m = n / 2 + 1;
+    } else if (nl_kind == RKS_NL_CUBIC_FFT) {
+      const int l2 = ilog2(n);
+      if (l2 < 4 || l2 > 13) {
+        rks_plan_destroy(p);
+        return fail(RKS_ERR_INVALID, "fft grid size must be a power of two in [16, 8192]");
+      }
+      m = n;
+    } else {
+      if (n <= 0) {
+        rks_plan_destroy(p);
+        return fail(RKS_ERR_INVALID, "number of modes must be positive");
+      }
+      m = n;
+    }
+    if (!desc[i].L) {
+      rks_plan_destroy(p);
+      return fail(RKS_ERR_INVALID, "lin_op pointer is null");
+    }
+    if (nl_kind == RKS_NL_ADVECT_RFFT && !desc[i].kx) {
+      rks_plan_destroy(p);
+      return fail(RKS_ERR_INVALID, "advective nonlinearity needs kx");
+    }
+    p->modes[i] = m;
+    p->ws_off[i] = ws_total;
+    ws_total += (size_t)(p->tab.nslot + 2 + p->tab.ncoef) * m;
+    Loff[i] = L_total;
+    L_total += (size_t)m * (desc[i].L_is_complex ? 2 : 1);
+    kxoff[i] = kx_total;
+    kx_total += (size_t)m;
+  }
+  HIPCHK(hipMalloc(&p->d_tab, sizeof(rks_tableau)));
+  HIPCHK(hipMalloc(&p->d_ctl, sizeof(rks_ctrl)));
+  HIPCHK(hipMalloc(&p->d_inst, sizeof(rks_inst) * ninst));
+  HIPCHK(hipMalloc(&p->d_io, sizeof(rks_io) * ninst));
+  HIPCHK(hipMalloc(&p->d_ws, ws_total * p->esize));
+  HIPCHK(hipMalloc(&p->d_L, L_total * sizeof(double)));
+  HIPCHK(hipMalloc(&p->d_kx, kx_total * (p->esize / 2)));
+  HIPCHK(hipMalloc(&p->d_queue, sizeof(int) * 64));
+  HIPCHK(hipMalloc(&p->d_order, sizeof(int) * ninst));
+  HIPCHK(hipMemcpy(p->d_tab, &p->tab, sizeof(rks_tableau), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(p->d_ctl, &p->ctl, sizeof(rks_ctrl), hipMemcpyHostToDevice));
+  HIPCHK(hipMemset(p->d_ws, 0, ws_total * p->esize));
+  double* d_kx64 = nullptr;
+  if (kx_total) HIPCHK(hipMalloc(&d_kx64, kx_total * sizeof(double)));
+  for (int i = 0; i < ninst; ++i) {
+    const int m = p->modes[i];
+    HIPCHK(hipMemcpy(p->d_L + Loff[i], desc[i].L, sizeof(double) * m * (desc[i].L_is_complex ? 2 : 1),
+                     hipMemcpyDefault));
+    if (desc[i].kx) HIPCHK(hipMemcpy(d_kx64 + kxoff[i], desc[i].kx, sizeof(double) * m, hipMemcpyDefault));
+    else HIPCHK(hipMemset(d_kx64 + kxoff[i], 0, sizeof(double) * m));
+  }
+  if (precision == RKS_F64) {
+    HIPCHK(hipMemcpy(p->d_kx, d_kx64, kx_total * sizeof(double), hipMemcpyDeviceToDevice));
+  } else if (kx_total) {
+    hipLaunchKernelGGL(rks_f64_to_f32, dim3(256), dim3(256), 0, 0, d_kx64, (float*)p->d_kx, (int)kx_total);
+    HIPCHK(hipGetLastError());
+  }
+  HIPCHK(hipDeviceSynchronize());
+  if (d_kx64) (void)hipFree(d_kx64);
+  // instance records
+  p->h_inst.assign(ninst, rks_inst());
+  p->h_io.assign(ninst, rks_io());
+  for (int i = 0; i < ninst; ++i) {
+    rks_inst& r = p->h_inst[i];
+    std::memset(&r, 0, sizeof(r));
+    r.n = desc[i].n;
+    r.m = p->modes[i];
+    r.L_complex = desc[i].L_is_complex ? 1 : 0;
+    r.L = p->d_L + Loff[i];
+    r.kx = (char*)p->d_kx + kxoff[i] * (p->esize / 2);
+    r.nl_coef = desc[i].nl_coef;
+    r.ws = (char*)p->d_ws + p->ws_off[i] * p->esize;
+    r.h_coeff = std::nan("");
+    r.ubuf = -1;
+    for (int s = 0; s < RKS_MAX_SLOTS; ++s) r.slot_map[s] = s;
+  }
+  HIPCHK(hipMemcpy(p->d_inst, p->h_inst.data(), sizeof(rks_inst) * ninst, hipMemcpyHostToDevice));
+  // buckets by size
+  if (nl_kind != RKS_NL_NONE) {
+    std::vector<std::vector<int>> byl(14);
+    for (int i = 0; i < ninst; ++i) byl[ilog2(desc[i].n)].push_back(i);
+    for (int l = 13; l >= 4; --l) {
+      if (byl[l].empty()) continue;
+      p->bucket_log2.push_back(l);
+      p->bucket_start.push_back((int)p->order.size());
+      p->bucket_count.push_back((int)byl[l].size());
+      for (int i : byl[l]) p->order.push_back(i);
+      KernelEntry e;
+      if (!lookup(precision, nl_kind, l, &e)) {
+        rks_plan_destroy(p);
+        return fail(RKS_ERR_INVALID, "no kernel for this size");
+      }
+      HIPCHK(hipFuncSetAttribute(e.fn, hipFuncAttributeMaxDynamicSharedMemorySize, e.lds));
+      const int N = 1 << l;
+      HIPCHK(hipMalloc(&p->d_tw[l], (size_t)N * p->esize));
+      if (precision == RKS_F64)
+        hipLaunchKernelGGL(rks_twiddle_kernel<double>, dim3((N + 255) / 256), dim3(256), 0, 0,
+                           (cx<double>*)p->d_tw[l], N);
+      else
+        hipLaunchKernelGGL(rks_twiddle_kernel<float>, dim3((N + 255) / 256), dim3(256), 0, 0,
+                           (cx<float>*)p->d_tw[l], N);
+      HIPCHK(hipGetLastError());
+    }
+    HIPCHK(hipMemcpy(p->d_order, p->order.data(), sizeof(int) * ninst, hipMemcpyHostToDevice));
+    HIPCHK(hipDeviceSynchronize());
+  }
+  *out = p;
+  return RKS_OK;
+}
+
+static void fill_ctrl(rks_ctrl* c, const rks_config* cfg) {
+  c->epsilon = cfg->epsilon;
+  c->incr_f = cfg->incr_f;
+  c->decr_f = cfg->decr_f;
+  c->safety_f = cfg->safety_f;
+  c->adapt_cutoff = cfg->adapt_cutoff;
+  c->minh = cfg->minh;
+  c->max_s = cfg->max_s;
+  c->min_s = cfg->min_s;
+  c->max_loops = cfg->max_loops;
+  c->contour_points = cfg->contour_points;
+  c->modecutoff = cfg->modecutoff;
+  c->contour_radius = cfg->contour_radius;
+}
+
+int rks_set_config(rks_plan* p, const rks_config* cfg, hipStream_t stream) {
+  if (!p || !cfg) return fail(RKS_ERR_INVALID, "null argument");
+  if (cfg->contour_points < 2) return fail(RKS_ERR_INVALID, "contour_points must be > 1");
+  fill_ctrl(&p->ctl, cfg);
+  HIPCHK(hipMemcpyAsync(p->d_ctl, &p->ctl, sizeof(rks_ctrl), hipMemcpyHostToDevice, stream));
+  HIPCHK(hipStreamSynchronize(stream));
+  return RKS_OK;
+}
+
+int rks_plan_destroy(rks_plan* p) {
+  if (!p) return RKS_OK;
+  (void)hipFree(p->d_tab);
+  (void)hipFree(p->d_ctl);
+  (void)hipFree(p->d_inst);
+  (void)hipFree(p->d_io);
+  (void)hipFree(p->d_ws);
+  (void)hipFree(p->d_L);
+  (void)hipFree(p->d_kx);
+  (void)hipFree(p->d_log);
+  (void)hipFree(p->d_snap);
+  (void)hipFree(p->d_snap_t);
+  (void)hipFree(p->d_queue);
+  (void)hipFree(p->d_order);
+  for (int l = 0; l < 14; ++l) (void)hipFree(p->d_tw[l]);
+  delete p;
+  return RKS_OK;
+}
+
+int rks_plan_info(const rks_plan* p, int* ninst, int* nslot, int* ncoef, int* esize) {
+  if (!p) return fail(RKS_ERR_INVALID, "null plan");
+  if (ninst) *ninst = p->ninst;
+  if (nslot) *nslot = p->tab.nslot;
+  if (ncoef) *ncoef = p->tab.ncoef;
+  if (esize) *esize = p->esize;
+  return RKS_OK;
+}
+
+int rks_set_storage(rks_plan* p, int store_freq, int snap_cap, int log_cap) {
+  if (!p) return fail(RKS_ERR_INVALID, "null plan");
+  if (store_freq < 0 || snap_cap < 0 || log_cap < 0) return fail(RKS_ERR_INVALID, "negative storage size");
+  if (store_freq > 0 && snap_cap < 2) return fail(RKS_ERR_INVALID, "snapshot capacity must be >= 2");
+  if (p->nlk == RKS_NL_NONE) return fail(RKS_ERR_INVALID, "storage needs a built-in nonlinearity");
+  HIPCHK(hipDeviceSynchronize());
+  (void)hipFree(p->d_snap);
+  (void)hipFree(p->d_snap_t);
+  (void)hipFree(p->d_log);
+  p->d_snap = nullptr;
+  p->d_snap_t = nullptr;
+  p->d_log = nullptr;
+  size_t tot_modes = 0;
+  for (int m : p->modes) tot_modes += (size_t)m;
+  if (store_freq > 0) {
+    HIPCHK(hipMalloc(&p->d_snap, tot_modes * snap_cap * p->esize));
+    HIPCHK(hipMalloc(&p->d_snap_t, sizeof(double) * snap_cap * p->ninst));
+  }
+  if (log_cap > 0) HIPCHK(hipMalloc(&p->d_log, sizeof(double) * 2 * (size_t)log_cap * p->ninst));
+  // patch the device records: read back state, set pointers, write back
+  HIPCHK(hipMemcpy(p->h_inst.data(), p->d_inst, sizeof(rks_inst) * p->ninst, hipMemcpyDeviceToHost));
+  size_t off = 0;
+  for (int i = 0; i < p->ninst; ++i) {
+    rks_inst& r = p->h_inst[i];
+    r.store_freq = store_freq;
+    r.snap_cap = store_freq > 0 ? snap_cap : 0;
+    r.snap = store_freq > 0 ? (char*)p->d_snap + off * snap_cap * p->esize : nullptr;
+    r.snap_t = store_freq > 0 ? p->d_snap_t + (size_t)snap_cap * i : nullptr;
+    r.log_cap = log_cap;
+    r.log = log_cap > 0 ? p->d_log + 2 * (size_t)log_cap * i : nullptr;
+    off += (size_t)p->modes[i];
+  }
+  HIPCHK(hipMemcpy(p->d_inst, p->h_inst.data(), sizeof(rks_inst) * p->ninst, hipMemcpyHostToDevice));
+  p->store_freq = store_freq;
+  p->snap_cap = snap_cap;
+  p->log_cap = log_cap;
+  return RKS_OK;
+}
+
+static int launch_all(rks_plan* p, int mode, double h_mode, hipStream_t stream) {
+  rks_launch L;
+  L.tab = p->d_tab;
+  L.ctl = p->d_ctl;
+  L.inst = p->d_inst;
+  L.io = p->d_io;
+  L.mode = mode;
+  L.h_mode = h_mode;
+  hipLaunchKernelGGL(rks_zero_int, dim3(1), dim3(64), 0, stream, p->d_queue, 64);
+  HIPCHK(hipGetLastError());
+  for (size_t b = 0; b < p->bucket_log2.size(); ++b) {
+    KernelEntry e;
+    lookup(p->prec, p->nlk, p->bucket_log2[b], &e);
+    L.order = p->d_order + p->bucket_start[b];
+    L.count = p->bucket_count[b];
+    L.queue = p->d_queue + b;
+    L.tw = p->d_tw[p->bucket_log2[b]];
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, e.fn, e.threads, e.lds) != hipSuccess || per_cu < 1)
+      per_cu = 1;
+    int grid = per_cu * p->cu;
+    if (grid > L.count) grid = L.count;
+    void* args[] = {&L};
+    HIPCHK(hipLaunchKernel(e.fn, dim3(grid), dim3(e.threads), args, (size_t)e.lds, stream));
+  }
+  return RKS_OK;
+}
+
+static int upload_io(rks_plan* p, const void* const* u_in, void* const* u_out, const double* t0,
+                     const double* tf, const double* h, int reset, int resume, hipStream_t stream) {
+  for (int i = 0; i < p->ninst; ++i) {
+    rks_io& o = p->h_io[i];
+    if (!resume) {
+      if (!u_in || !u_out || !u_in[i] || !u_out[i]) return fail(RKS_ERR_INVALID, "null state pointer");
+      if (u_in[i] == u_out[i]) return fail(RKS_ERR_INVALID, "u_in and u_out must not alias");
+      o.u_in = u_in[i];
+      o.u_out = u_out[i];
+      o.t0 = t0 ? t0[i] : 0.0;
+      o.tf = tf ? tf[i] : 0.0;
+      o.h_in = h ? h[i] : 0.0;
+    }
+    o.reset = reset;
+    o.resume = resume;
+  }
+  HIPCHK(hipMemcpyAsync(p->d_io, p->h_io.data(), sizeof(rks_io) * p->ninst, hipMemcpyHostToDevice, stream));
+  return RKS_OK;
+}
+
+int rks_evolve(rks_plan* p, const void* const* u0, void* const* uf, const double* t0, const double* tf,
+               const double* h, hipStream_t stream) {
+  if (!p) return fail(RKS_ERR_INVALID, "null plan");
+  if (p->nlk == RKS_NL_NONE) return fail(RKS_ERR_INVALID, "evolve needs a built-in nonlinearity");
+  for (int i = 0; i < p->ninst; ++i) {
+    if (!(h && h[i] >= 0.0)) return fail(RKS_ERR_INVALID, "step size must be >= 0");
+    if (!p->tab.adaptive && t0[i] < tf[i] && !(h[i] > 0.0))
+      return fail(RKS_ERR_INVALID, "constant-step evolve needs h > 0");
+    if (!p->tab.adaptive && (tf[i] - t0[i]) / h[i] > 1e9)
+      return fail(RKS_ERR_INVALID, "constant-step evolve would take more than 1e9 steps");
+  }
+  int rc = upload_io(p, u0, uf, t0, tf, h, 1, 0, stream);
+  if (rc) return rc;
+  return launch_all(p, RKS_MODE_EVOLVE, 0.0, stream);
+}
+
+int rks_resume(rks_plan* p, hipStream_t stream) {
+  if (!p) return fail(RKS_ERR_INVALID, "null plan");
+  int rc = upload_io(p, nullptr, nullptr, nullptr, nullptr, nullptr, 0, 1, stream);
+  if (rc) return rc;
+  return launch_all(p, RKS_MODE_EVOLVE, 0.0, stream);
+}
+
+int rks_step(rks_plan* p, const void* const* u, void* const* u_new, const double* h, hipStream_t stream) {
+  if (!p) return fail(RKS_ERR_INVALID, "null plan");
+  if (p->nlk == RKS_NL_NONE) return fail(RKS_ERR_INVALID, "step needs a built-in nonlinearity");
+  for (int i = 0; i < p->ninst; ++i)
+    if (!(h && h[i] >= 0.0)) return fail(RKS_ERR_INVALID, "step size must be >= 0");
+  int rc = upload_io(p, u, u_new, nullptr, nullptr, h, 0, 0, stream);
+  if (rc) return rc;
+  return launch_all(p, RKS_MODE_STEP, 0.0, stream);
+}
+
+int rks_reset(rks_plan* p, hipStream_t stream) {
+  if (!p) return fail(RKS_ERR_INVALID, "null plan");
+  HIPCHK(hipStreamSynchronize(stream));
+  HIPCHK(hipMemcpy(p->h_inst.data(), p->d_inst, sizeof(rks_inst) * p->ninst, hipMemcpyDeviceToHost));
+  for (auto& r : p->h_inst) {
+    r.started = 0;
+    r.accept = 0;
+    r.h_coeff = std::nan("");
+    r.ubuf = -1;
+    r.status = RKS_ST_OK;
+    for (int s = 0; s < RKS_MAX_SLOTS; ++s) r.slot_map[s] = s;
+  }
+  HIPCHK(hipMemcpy(p->d_inst, p->h_inst.data(), sizeof(rks_inst) * p->ninst, hipMemcpyHostToDevice));
+  return RKS_OK;
+}
+
+int rks_nl_eval(rks_plan* p, const void* const* u, void* const* nl_out, hipStream_t stream) {
+  if (!p) return fail(RKS_ERR_INVALID, "null plan");
+  if (p->nlk == RKS_NL_NONE) return fail(RKS_ERR_INVALID, "no built-in nonlinearity");
+  int rc = upload_io(p, u, nl_out, nullptr, nullptr, nullptr, 0, 0, stream);
+  if (rc) return rc;
+  return launch_all(p, RKS_MODE_NL, 0.0, stream);
+}
+
+int rks_coeffs(rks_plan* p, int inst, double h, void* dst, hipStream_t stream) {
+  if (!p || inst < 0 || inst >= p->ninst) return fail(RKS_ERR_INVALID, "bad plan or instance");
+  const int M = p->modes[inst];
+  const size_t nbytes = (size_t)p->tab.ncoef * M * p->esize;
+  const rks_inst& r = p->h_inst[inst];
+  void* coefs = (char*)r.ws + (size_t)(p->tab.nslot + 2) * M * p->esize;
+  const int grid = (M + 255) / 256;
+  if (p->prec == RKS_F64)
+    hipLaunchKernelGGL(rks_coef_kernel<double>, dim3(grid), dim3(256), 0, stream, p->d_tab, p->d_ctl, r.L,
+                       r.L_complex, M, h, (cx<double>*)coefs);
+  else
+    hipLaunchKernelGGL(rks_coef_kernel<float>, dim3(grid), dim3(256), 0, stream, p->d_tab, p->d_ctl, r.L,
+                       r.L_complex, M, h, (cx<float>*)coefs);
+  HIPCHK(hipGetLastError());
+  if (dst) HIPCHK(hipMemcpyAsync(dst, coefs, nbytes, hipMemcpyDeviceToDevice, stream));
+  return RKS_OK;
+}
+
+int rks_combine(rks_plan* p, int inst, int stage, double h, const void* u, const void* const* slots, void* out,
+                hipStream_t stream) {
+  if (!p || inst < 0 || inst >= p->ninst) return fail(RKS_ERR_INVALID, "bad plan or instance");
+  const rks_tableau& tb = p->tab;
+  const rks_term* terms;
+  int nterm;
+  const bool is_err = (stage == -1);
+  if (stage >= 0 && stage < tb.nstage) {
+    terms = tb.st[stage].t;
+    nterm = tb.st[stage].nterm;
+  } else if (is_err && tb.adaptive) {
+    terms = tb.err;
+    nterm = tb.nerr;
+  } else {
+    return fail(RKS_ERR_INVALID, "bad stage index");
+  }
+  if (!out) return fail(RKS_ERR_INVALID, "null output");
+  if (!is_err && !u) return fail(RKS_ERR_INVALID, "null u");
+  rks_combine_args A;
+  std::memset(&A, 0, sizeof(A));
+  for (int s = 0; s < tb.nslot && s < RKS_MAX_SLOTS; ++s) A.slots[s] = slots ? slots[s] : nullptr;
+  for (int j = 0; j < nterm; ++j) {
+    if (!A.slots[terms[j].src]) return fail(RKS_ERR_INVALID, "missing NL slot for this stage");
+    if (terms[j].src2 >= 0 && !A.slots[terms[j].src2]) return fail(RKS_ERR_INVALID, "missing NL slot for this stage");
+  }
+  const int M = p->modes[inst];
+  const rks_inst& r = p->h_inst[inst];
+  A.tab = p->d_tab;
+  A.u = u;
+  A.coefs = (char*)r.ws + (size_t)(tb.nslot + 2) * M * p->esize;
+  A.out = out;
+  A.stage = stage;
+  A.is_err = is_err ? 1 : 0;
+  A.M = M;
+  A.h = h;
+  int grid = (M + 255) / 256;
+  if (grid > 4 * p->cu) grid = 4 * p->cu;
+  if (p->prec == RKS_F64) hipLaunchKernelGGL(rks_combine_kernel<double>, dim3(grid), dim3(256), 0, stream, A);
+  else hipLaunchKernelGGL(rks_combine_kernel<float>, dim3(grid), dim3(256), 0, stream, A);
+  HIPCHK(hipGetLastError());
+  return RKS_OK;
+}
+
+int rks_error_s(rks_plan* p, int inst, const void* u_new, const void* err, double* s_dev, hipStream_t stream) {
+  if (!p || inst < 0 || inst >= p->ninst || !u_new || !err || !s_dev)
+    return fail(RKS_ERR_INVALID, "bad arguments");
+  const int M = p->modes[inst];
+  if (p->prec == RKS_F64)
+    hipLaunchKernelGGL(rks_norm_kernel<double>, dim3(1), dim3(1024), 0, stream, (const cx<double>*)u_new,
+                       (const cx<double>*)err, M, p->ctl.epsilon, p->ctl.safety_f, p->ctl.adapt_cutoff,
+                       p->tab.q, s_dev);
+  else
+    hipLaunchKernelGGL(rks_norm_kernel<float>, dim3(1), dim3(1024), 0, stream, (const cx<float>*)u_new,
+                       (const cx<float>*)err, M, p->ctl.epsilon, p->ctl.safety_f, p->ctl.adapt_cutoff,
+                       p->tab.q, s_dev);
+  HIPCHK(hipGetLastError());
+  return RKS_OK;
+}
+
+int rks_get_stats(rks_plan* p, rks_stats* out, hipStream_t stream) {
+  if (!p || !out) return fail(RKS_ERR_INVALID, "null argument");
+  HIPCHK(hipStreamSynchronize(stream));
+  HIPCHK(hipMemcpy(p->h_inst.data(), p->d_inst, sizeof(rks_inst) * p->ninst, hipMemcpyDeviceToHost));
+  for (int i = 0; i < p->ninst; ++i) {
+    const rks_inst& r = p->h_inst[i];
+    rks_stats& s = out[i];
+    s.tc = r.tc;
+    s.h_used = r.h_used;
+    s.h_next = r.h_next;
+    s.h = r.h;
+    s.steps = r.steps;
+    s.attempts = r.attempts;
+    s.rejects = r.rejects;
+    s.status = r.status;
+    s.log_count = r.log_count;
+    s.snap_count = r.snap_count;
+    s.nl_calls = r.nl_calls;
+    s.coef_updates = r.coef_updates;
+  }
+  return RKS_OK;
+}
+
+int rks_read_log(rks_plan* p, int inst, double* host_hs, int cap, hipStream_t stream) {
+  if (!p || inst < 0 || inst >= p->ninst || !host_hs) return fail(RKS_ERR_INVALID, "bad arguments");
+  if (!p->d_log) return fail(RKS_ERR_INVALID, "attempt log not enabled");
+  HIPCHK(hipStreamSynchronize(stream));
+  int n = cap < p->log_cap ? cap : p->log_cap;
+  HIPCHK(hipMemcpy(host_hs, p->d_log + 2 * (size_t)p->log_cap * inst, sizeof(double) * 2 * n,
+                   hipMemcpyDeviceToHost));
+  return RKS_OK;
+}
+
+int rks_read_snapshots(rks_plan* p, int inst, void* dst_dev, double* t_host, int count, hipStream_t stream) {
+  if (!p || inst < 0 || inst >= p->ninst) return fail(RKS_ERR_INVALID, "bad arguments");
+  if (!p->d_snap) return fail(RKS_ERR_INVALID, "snapshots not enabled");
+  if (count < 0 || count > p->snap_cap) return fail(RKS_ERR_INVALID, "bad snapshot count");
+  const rks_inst& r = p->h_inst[inst];
+  if (count == 0) return RKS_OK;
+  if (dst_dev)
+    HIPCHK(hipMemcpyAsync(dst_dev, r.snap, (size_t)count * r.m * p->esize, hipMemcpyDeviceToDevice, stream));
+  if (t_host) {
+    HIPCHK(hipStreamSynchronize(stream));
+    HIPCHK(hipMemcpy(t_host, r.snap_t, sizeof(double) * count, hipMemcpyDeviceToHost));
+  }
+  // consumed: the next resume starts filling at index 0
+  HIPCHK(hipStreamSynchronize(stream));
+  HIPCHK(hipMemcpy(p->h_inst.data(), p->d_inst, sizeof(rks_inst) * p->ninst, hipMemcpyDeviceToHost));
+  p->h_inst[inst].snap_count = 0;
+  HIPCHK(hipMemcpy(p->d_inst + inst, &p->h_inst[inst], sizeof(rks_inst), hipMemcpyHostToDevice));
+  return RKS_OK;
+}
+
+}  // extern "C"

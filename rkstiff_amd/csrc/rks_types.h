@@ -1,0 +1,151 @@
+// Shared host/device descriptors of the stepping engine.
+//
+// A method is described by a tableau: which coefficient arrays it needs (each a
+// rational combination of h*psi_r(f z) or exp(f z) basis values, z = h*L), and
+// for every stage the linear combination k = E .* u + sum_j scale_j h^p_j C_j .* NL_j
+// that feeds the nonlinear term.  All seven diagonal methods of the reference
+// (etd35.py, etd34.py, etd4.py, etd5.py, if34.py, if45dp.py, if4.py) are
+// encoded as tableaux in rks_tableau.cpp; the kernels are method-agnostic.
+#pragma once
+#include <stdint.h>
+
+#define RKS_MAX_STAGES 8
+#define RKS_MAX_TERMS 8
+#define RKS_MAX_COEF 32
+#define RKS_MAX_BASIS 16
+#define RKS_MAX_SLOTS 8
+#define RKS_COEF_ONE (-1)
+
+// basis value kinds (rks_basis.r)
+#define RKS_BASIS_EXP 0  // exp(f z)
+#define RKS_BASIS_PSI1 1 // h * psi1(f z)   (psi_r = r! phi_r)
+#define RKS_BASIS_PSI2 2
+#define RKS_BASIS_PSI3 3
+
+// how NL1 is obtained after an accepted attempt
+#define RKS_NL1_FSAL 0       // NL1 <- NL(last stage)   (IF34, ETD34, IF45DP, constant methods)
+#define RKS_NL1_RECOMPUTE 1  // NL1 <- nl(u)            (ETD35, etd35.py:317-318)
+
+// instance status
+#define RKS_ST_OK 0
+#define RKS_ST_MAXLOOPS 3
+#define RKS_ST_MINSTEP 4
+#define RKS_ST_PAUSED 5  // snapshot buffer full; evolve can be resumed
+#define RKS_ST_RUNNING 6
+
+// kernel modes
+#define RKS_MODE_EVOLVE 0
+#define RKS_MODE_STEP 1
+#define RKS_MODE_NL 2      // NL = nl(u) only
+#define RKS_MODE_COEFFS 3  // coefficient arrays at h only
+
+// One term of a stage combination, in the reference's grouping:
+//   inner = NL[src] + scale2 * NL[src2]                      (src2 < 0: inner = NL[src])
+//   hmode 0: scale * (C * inner)          e.g. a52 * (NL2 - NL3)      (etd35.py:326)
+//   hmode 1: ((h * C) * inner) * scale    e.g. h * EL2 * NL1 / 2.0    (if34.py:120)
+//   hmode 2: ((scale * h) / den) * inner  e.g. 11 * h / 84.0 * NL6    (if45dp.py:224)
+typedef struct rks_term {
+  int src;
+  int src2;
+  int coef;      // coefficient array index, or RKS_COEF_ONE
+  int hmode;
+  double scale;
+  double scale2;
+  double den;
+} rks_term;
+
+typedef struct rks_stage {
+  int ecoef;   // coefficient multiplying u (RKS_COEF_ONE allowed)
+  int nterm;
+  int has_nl;  // evaluate NL[nl_slot] = nl(k)
+  int nl_slot;
+  int hsum;    // k = E u + h * (sum of terms)   (if34.py:126-128)
+  int pad;
+  rks_term t[RKS_MAX_TERMS];
+} rks_stage;
+
+typedef struct rks_basis {
+  int r;       // RKS_BASIS_*
+  int fnum;    // argument f z with f = fnum / fden
+  int fden;
+  int pad;
+} rks_basis;
+
+typedef struct rks_coef {
+  int nb;
+  int hmul;            // weights multiplied by h (IF45DP a_ij = w h exp(f z))
+  int basis[4];
+  double w[4];
+  double outer;        // c = (outer * (sum_k w_k B_k)) / den
+  double den;
+} rks_coef;
+
+typedef struct rks_tableau {
+  int method;
+  int adaptive;
+  int q;
+  int nl1_policy;
+  int fsal_slot;       // slot that becomes NL1 on accept (RKS_NL1_FSAL)
+  int nslot;
+  int nstage;
+  int nerr;
+  int ncoef;
+  int nbasis;
+  int uses_contour;    // ETD methods: small modes through the contour mean
+  int err_coef;        // factored error estimate: err = C[err_coef] * (((h^err_hpow) * sum(terms)) * err_scale)
+  int err_factored;    //   (0: err = sum of the terms)
+  int err_hpow;
+  int pad2;
+  double err_scale;
+  rks_stage st[RKS_MAX_STAGES];
+  rks_term err[RKS_MAX_TERMS];
+  rks_coef coef[RKS_MAX_COEF];
+  rks_basis basis[RKS_MAX_BASIS];
+} rks_tableau;
+
+typedef struct rks_ctrl {
+  // SolverConfig (solveras.py:71-94)
+  double epsilon, incr_f, decr_f, safety_f, adapt_cutoff, minh;
+  // class constants (solveras.py:255-257)
+  double max_s, min_s;
+  int max_loops;
+  // ETDConfig (etd.py:81)
+  int contour_points;
+  double modecutoff, contour_radius;
+} rks_ctrl;
+
+// Per-instance record in device memory: geometry, buffers and the persistent
+// solver state.  The persistent part lets an evolve pause (snapshot buffer
+// full) and resume, and lets step() keep FSAL / coefficient-cache state
+// between calls exactly like the reference's solver object.  Only the kernels
+// write the state; the host writes the record once at plan creation.
+typedef struct rks_inst {
+  int n;               // real-space points
+  int m;               // spectral modes
+  int L_complex;       // L stored as complex128 (1) or float64 (0)
+  int log_cap;         // attempt-log capacity (entries of 2 doubles)
+  const double* L;     // FP64 lin_op
+  const void* kx;      // wavenumbers in engine precision (advect nl)
+  double nl_coef;
+  void* ws;            // workspace (engine precision complex)
+  double* log;         // attempt log: (h, s) pairs
+  void* snap;          // snapshot buffer (snap_cap * m complex)
+  double* snap_t;
+  int snap_cap;
+  int store_freq;
+  // persistent state
+  double tc, tf, h, h_coeff, h_used, h_next;
+  int started, accept, steps, attempts;
+  int rejects, status, ubuf, numloops;  // ubuf: -1 = caller's u_in, 0/1 = candidate buffers
+  int log_count, snap_count, nl_calls, coef_updates;
+  int slot_map[RKS_MAX_SLOTS];
+} rks_inst;
+
+// Per-launch inputs, written by the host before every launch.
+typedef struct rks_io {
+  const void* u_in;    // state at t0 (evolve) or u (step); never written
+  void* u_out;         // result
+  double t0, tf, h_in; // h_in: first step (evolve) / h_suggest (step)
+  int reset;           // clear solver state first (evolve, reset())
+  int resume;          // evolve: continue a paused trajectory
+} rks_io;

@@ -1,0 +1,193 @@
+"""Plan: a Python handle on one C-ABI plan (rks_plan_create ... rks_plan_destroy).
+
+A plan holds ``ninst`` trajectories sharing one method, nonlinearity and
+precision; each instance has its own grid size, lin_op and wavenumbers.  All
+device work is done by librkstiff_amd.so on the current torch stream.
+"""
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib
+from .config import ETDConfig, SolverConfig
+
+STATUS_NAMES = {0: "OK", 3: "MAXLOOPS", 4: "MINSTEP", 5: "PAUSED", 6: "RUNNING"}
+
+
+def _stream():
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def make_config(config=None, etd_config=None, max_loops=50, max_s=4.0, min_s=0.25):
+    config = config if config is not None else SolverConfig()
+    etd_config = etd_config if etd_config is not None else ETDConfig()
+    c = _lib.Config()
+    c.epsilon, c.incr_f, c.decr_f = config.epsilon, config.incr_f, config.decr_f
+    c.safety_f, c.adapt_cutoff, c.minh = config.safety_f, config.adapt_cutoff, config.minh
+    c.max_s, c.min_s, c.max_loops = float(max_s), float(min_s), int(max_loops)
+    c.contour_points = int(etd_config.contour_points)
+    c.modecutoff = float(etd_config.modecutoff)
+    c.contour_radius = float(etd_config.contour_radius)
+    return c
+
+
+def config_key(c):
+    return tuple(getattr(c, f) for f, _ in c._fields_)
+
+
+def lin_op_f64(L):
+    """lin_op as (float64 or complex128 torch tensor on CPU, is_complex)."""
+    if isinstance(L, torch.Tensor):
+        t = L.detach().cpu()
+    else:
+        t = torch.as_tensor(np.asarray(L))
+    if t.is_complex():
+        return t.to(torch.complex128).contiguous(), True
+    return t.to(torch.float64).contiguous(), False
+
+
+class Plan:
+    """ninst trajectories of one method / nonlinearity / precision on the current device."""
+
+    def __init__(self, method, nl, precision, config, etd_config, descs, max_loops=50, max_s=4.0,
+                 min_s=0.25):
+        lib = _lib.load()
+        if not torch.cuda.is_available():
+            raise RuntimeError("rkstiff_amd needs a HIP device (torch.cuda.is_available() is False)")
+        self.lib = lib
+        self.method = method
+        self.precision = precision
+        self.nl_kind = _lib.NL_NONE if nl is None else nl.kind
+        self.device = torch.cuda.current_device()
+        self.cfg = make_config(config, etd_config, max_loops, max_s, min_s)
+        self._cfg_key = config_key(self.cfg)
+        arr = (_lib.InstanceDesc * len(descs))()
+        keep = []
+        self.modes = []
+        for i, d in enumerate(descs):
+            L, cplx = lin_op_f64(d["L"])
+            Lv = torch.view_as_real(L).reshape(-1) if cplx else L
+            keep.append(Lv)
+            arr[i].n = int(d["n"])
+            arr[i].L_is_complex = 1 if cplx else 0
+            arr[i].L = Lv.data_ptr()
+            if d.get("kx") is not None:
+                kx = torch.as_tensor(d["kx"], dtype=torch.float64).cpu().contiguous()
+                keep.append(kx)
+                arr[i].kx = kx.data_ptr()
+            else:
+                arr[i].kx = None
+            arr[i].nl_coef = float(d.get("nl_coef", 1.0))
+            m = L.numel()
+            self.modes.append(m)
+        handle = ctypes.c_void_p()
+        _lib.check(lib.rks_plan_create(_lib.METHOD_IDS[method], self.nl_kind, precision,
+                                       ctypes.byref(self.cfg), len(descs), arr, ctypes.byref(handle)))
+        self.h = handle
+        self.ninst = len(descs)
+        nslot, ncoef, es = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        _lib.check(lib.rks_plan_info(self.h, None, ctypes.byref(nslot), ctypes.byref(ncoef),
+                                     ctypes.byref(es)))
+        self.nslot, self.ncoef, self.esize = nslot.value, ncoef.value, es.value
+        self.dtype = torch.complex128 if precision == _lib.F64 else torch.complex64
+        self.storage = (0, 0, 0)
+
+    def __del__(self):
+        h = getattr(self, "h", None)
+        if h is not None and h.value:
+            try:
+                self.lib.rks_plan_destroy(h)
+            except Exception:
+                pass
+            self.h = None
+
+    # -- configuration -------------------------------------------------------
+    def set_config(self, config, etd_config, max_loops, max_s, min_s):
+        c = make_config(config, etd_config, max_loops, max_s, min_s)
+        key = config_key(c)
+        if key != self._cfg_key:
+            _lib.check(self.lib.rks_set_config(self.h, ctypes.byref(c), _stream()))
+            self.cfg, self._cfg_key = c, key
+
+    def set_storage(self, store_freq, snap_cap, log_cap):
+        if (store_freq, snap_cap, log_cap) != self.storage:
+            torch.cuda.current_stream().synchronize()
+            _lib.check(self.lib.rks_set_storage(self.h, store_freq, snap_cap, log_cap))
+            self.storage = (store_freq, snap_cap, log_cap)
+
+    # -- launches --------------------------------------------------------------
+    def _check_state(self, tensors):
+        for i, t in enumerate(tensors):
+            if t.dtype != self.dtype or not t.is_cuda or not t.is_contiguous():
+                raise ValueError(f"state {i}: expected contiguous {self.dtype} on the device")
+            if t.numel() != self.modes[i]:
+                raise ValueError(f"state {i}: expected {self.modes[i]} modes, got {t.numel()}")
+
+    def evolve(self, u0, uf, t0, tf, h):
+        self._check_state(u0)
+        self._check_state(uf)
+        _lib.check(self.lib.rks_evolve(self.h, _lib.ptr_array([t.data_ptr() for t in u0]),
+                                       _lib.ptr_array([t.data_ptr() for t in uf]),
+                                       _lib.dbl_array(t0), _lib.dbl_array(tf), _lib.dbl_array(h),
+                                       _stream()))
+
+    def resume(self):
+        _lib.check(self.lib.rks_resume(self.h, _stream()))
+
+    def step(self, u, unew, h):
+        self._check_state(u)
+        self._check_state(unew)
+        _lib.check(self.lib.rks_step(self.h, _lib.ptr_array([t.data_ptr() for t in u]),
+                                     _lib.ptr_array([t.data_ptr() for t in unew]),
+                                     _lib.dbl_array(h), _stream()))
+
+    def reset(self):
+        _lib.check(self.lib.rks_reset(self.h, _stream()))
+
+    def nl_eval(self, u, out):
+        self._check_state(u)
+        self._check_state(out)
+        _lib.check(self.lib.rks_nl_eval(self.h, _lib.ptr_array([t.data_ptr() for t in u]),
+                                        _lib.ptr_array([t.data_ptr() for t in out]), _stream()))
+
+    def coeffs(self, inst, h, out=None):
+        """Coefficient arrays (ncoef x m) of instance ``inst`` at step h."""
+        if out is None:
+            out = torch.empty((self.ncoef, self.modes[inst]), dtype=self.dtype, device="cuda")
+        _lib.check(self.lib.rks_coeffs(self.h, inst, float(h), ctypes.c_void_p(out.data_ptr()),
+                                       _stream()))
+        return out
+
+    def combine(self, inst, stage, h, u, slots, out):
+        ptrs = [0 if s is None else s.data_ptr() for s in slots]
+        ptrs += [0] * (_lib.MAX_SLOTS - len(ptrs))
+        _lib.check(self.lib.rks_combine(self.h, inst, stage, float(h),
+                                        ctypes.c_void_p(u.data_ptr() if u is not None else 0),
+                                        _lib.ptr_array(ptrs), ctypes.c_void_p(out.data_ptr()),
+                                        _stream()))
+
+    def error_s(self, inst, unew, err, s_dev):
+        _lib.check(self.lib.rks_error_s(self.h, inst, ctypes.c_void_p(unew.data_ptr()),
+                                        ctypes.c_void_p(err.data_ptr()),
+                                        ctypes.c_void_p(s_dev.data_ptr()), _stream()))
+
+    # -- results ---------------------------------------------------------------
+    def stats(self):
+        arr = (_lib.Stats * self.ninst)()
+        _lib.check(self.lib.rks_get_stats(self.h, arr, _stream()))
+        return [{f: getattr(a, f) for f, _ in _lib.Stats._fields_} for a in arr]
+
+    def attempt_log(self, inst, count):
+        count = min(count, self.storage[2])
+        buf = (ctypes.c_double * (2 * max(count, 1)))()
+        _lib.check(self.lib.rks_read_log(self.h, inst, buf, count, _stream()))
+        return np.frombuffer(buf, dtype=np.float64)[: 2 * count].reshape(count, 2).copy()
+
+    def snapshots(self, inst, count):
+        """Drain ``count`` stored snapshots of ``inst``: (times, tensor[count, m])."""
+        out = torch.empty((count, self.modes[inst]), dtype=self.dtype, device="cuda")
+        tbuf = (ctypes.c_double * max(count, 1))()
+        _lib.check(self.lib.rks_read_snapshots(self.h, inst, ctypes.c_void_p(out.data_ptr()), tbuf,
+                                               count, _stream()))
+        return list(np.frombuffer(tbuf, dtype=np.float64)[:count]), out

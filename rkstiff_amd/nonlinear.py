@@ -1,0 +1,130 @@
+"""Built-in nonlinear terms and model factories.
+
+A built-in term is a callable ``nl(u)`` on device tensors (evaluated by the
+HIP library, ``rks_nl_eval``) that the solvers also recognise: with a built-in
+term the whole stage -- combination, inverse FFT, pointwise product, forward
+FFT -- runs fused inside the stepping kernel.  Any other callable works too,
+through the generic path (stage combinations and error norm on the device, the
+callable evaluated between them).
+
+Terms follow the reference's ``nl_func`` patterns:
+
+``AdvectiveNL(kx, coef)``  coef * rfft(irfft(u) * irfft(1j*kx*u))
+    KS and Burgers: coef = -1 (README.md:94-97, models.py:189-192);
+    KdV: coef = -6 (models.py:140-143).
+``CubicNL(n, gamma)``      1j*gamma*fft(|v|^2 v), v = ifft(u)   (demos/nls.ipynb cell 29)
+"""
+import numpy as np
+import torch
+
+from . import _lib
+
+
+def _as_f64(x):
+    if isinstance(x, torch.Tensor):
+        return x.detach().to(torch.float64).cpu().contiguous()
+    return torch.as_tensor(np.asarray(x, dtype=np.float64))
+
+
+class BuiltinNL:
+    """Base class: a nonlinear term the stepping kernels evaluate in LDS."""
+
+    kind = None
+
+    def __init__(self):
+        self._plans = {}
+
+    def n_points(self):
+        raise NotImplementedError
+
+    def modes(self):
+        raise NotImplementedError
+
+    def descriptor(self, L=None):
+        raise NotImplementedError
+
+    def __call__(self, u):
+        from .engine import Plan
+        if not isinstance(u, torch.Tensor) or not u.is_cuda:
+            raise TypeError("built-in nonlinear terms evaluate device (HIP) tensors")
+        if u.dtype not in (torch.complex128, torch.complex64):
+            raise TypeError("state must be complex128 or complex64")
+        if u.shape[-1] != self.modes():
+            raise ValueError(f"expected {self.modes()} modes, got {u.shape[-1]}")
+        prec = _lib.F64 if u.dtype == torch.complex128 else _lib.F32
+        key = (prec, u.device.index)
+        if key not in self._plans:
+            L = torch.zeros(self.modes(), dtype=torch.float64)
+            self._plans[key] = Plan("IF4", self, prec, None, None, [self.descriptor(L)])
+        u = u.contiguous()
+        out = torch.empty_like(u)
+        self._plans[key].nl_eval([u], [out])
+        return out
+
+
+class AdvectiveNL(BuiltinNL):
+    """coef * rfft(u * u_x) for real fields on an rfft grid of n points (kx has n//2+1 entries)."""
+
+    kind = _lib.NL_ADVECT_RFFT
+
+    def __init__(self, kx, coef=-1.0):
+        super().__init__()
+        self.kx = _as_f64(kx)
+        self.coef = float(coef)
+        self.n = 2 * (self.kx.numel() - 1)
+
+    def n_points(self):
+        return self.n
+
+    def modes(self):
+        return self.n // 2 + 1
+
+    def descriptor(self, L):
+        return dict(n=self.n, L=L, kx=self.kx, nl_coef=self.coef)
+
+
+class CubicNL(BuiltinNL):
+    """1j*gamma*fft(|v|^2 v), v = ifft(u), on an fft grid of n points."""
+
+    kind = _lib.NL_CUBIC_FFT
+
+    def __init__(self, n, gamma=1.0):
+        super().__init__()
+        self.n = int(n)
+        self.gamma = float(gamma)
+
+    def n_points(self):
+        return self.n
+
+    def modes(self):
+        return self.n
+
+    def descriptor(self, L):
+        return dict(n=self.n, L=L, kx=None, nl_coef=self.gamma)
+
+
+# ---------------------------------------------------------------------------
+# model factories: (lin_op, nl_func) pairs, as in the reference's models.py
+# ---------------------------------------------------------------------------
+def ks_ops(kx):
+    """Kuramoto-Sivashinsky: L = kx^2 (1 - kx^2), N = -rfft(u u_x) (README.md:89-97)."""
+    kx = _as_f64(kx)
+    return kx**2 * (1 - kx**2), AdvectiveNL(kx, -1.0)
+
+
+def kdv_ops(kx):
+    """KdV: L = i kx^3, N = -6 rfft(u u_x) (models.py:113-145)."""
+    kx = _as_f64(kx)
+    return torch.complex(torch.zeros_like(kx), kx**3), AdvectiveNL(kx, -6.0)
+
+
+def burgers_ops(kx, mu):
+    """Viscous Burgers: L = -mu kx^2, N = -rfft(u u_x) (models.py:153-194)."""
+    kx = _as_f64(kx)
+    return -mu * kx**2, AdvectiveNL(kx, -1.0)
+
+
+def nls_ops(kx, gamma=1.0):
+    """Cubic NLS u_t = (i/2) u_xx + i gamma |u|^2 u: L = -(i/2) kx^2, N = 1j gamma fft(|v|^2 v)."""
+    kx = _as_f64(kx)
+    return torch.complex(torch.zeros_like(kx), -0.5 * kx**2), CubicNL(kx.numel(), gamma)

@@ -1,0 +1,497 @@
+"""Drop-in solver classes on PyTorch-ROCm device tensors.
+
+Same constructors, ``step``, ``evolve``, ``reset``, ``t``/``u`` storage and
+exceptions as the reference's solver classes:
+
+  adaptive  ETD35 (etd35.py:723-889), ETD34 (etd34.py:439-621), IF34 (if34.py:260-338),
+            IF45DP (if45dp.py:8-249) on BaseSolverAS (solveras.py:175-650)
+  constant  ETD4 (etd4.py:311-422), ETD5 (etd5.py:485-655), IF4 (if4.py:202-275)
+            on BaseSolverCS (solvercs.py:31-279)
+
+Two paths, both on the device:
+
+* built-in nonlinear term (``rkstiff_amd.nonlinear``): ``evolve`` is one launch
+  of the persistent stepping kernel (stages, FFTs, error norm and the
+  accept/reject controller all on the device); ``step`` is one launch plus a
+  read of (h_used, h_next).
+* any other callable ``nl_func(u) -> tensor``: coefficients, stage combinations
+  and the error norm run as library kernels, the callable runs between them,
+  and the controller runs here with the reference's arithmetic.
+
+Only diagonal (1-D) ``lin_op`` is supported; a square 2-D ``lin_op`` (the
+reference's dense/diagonalized strategies) raises NotImplementedError.
+"""
+import logging
+import math
+
+import numpy as np
+import torch
+
+from . import _lib
+from .config import ETDConfig, SolverConfig, SolverType
+from .engine import Plan, STATUS_NAMES
+from .nonlinear import BuiltinNL
+
+_DEFAULT_CONFIG = SolverConfig()
+_DEFAULT_ETD = ETDConfig()
+
+LEVELS = {"DEBUG": logging.DEBUG, "INFO": logging.INFO, "WARNING": logging.WARNING,
+          "ERROR": logging.ERROR, "CRITICAL": logging.CRITICAL}
+
+
+def _level(loglevel):
+    if isinstance(loglevel, int):
+        return loglevel
+    if str(loglevel).upper() not in LEVELS:
+        raise ValueError(f"Invalid log level: {loglevel}")
+    return LEVELS[str(loglevel).upper()]
+
+
+class BaseSolver:
+    """lin_op / nl_func holder, solver.py:91-136."""
+
+    method = None
+
+    def __init__(self, lin_op, nl_func, loglevel="WARNING"):
+        self.lin_op = lin_op
+        self.nl_func = nl_func
+        self.logger = logging.getLogger(f"rkstiff_amd.{self.__class__.__name__}")
+        self.logger.setLevel(_level(loglevel))
+        self.t, self.u = [], []
+        dims = tuple(lin_op.shape)
+        if len(dims) not in (1, 2):
+            raise ValueError("lin_op must be 1D or 2D")
+        if len(dims) == 2 and dims[0] != dims[1]:
+            raise ValueError("lin_op must be a square matrix")
+        self._diag = len(dims) == 1
+        if not self._diag:
+            raise NotImplementedError("rkstiff_amd steps diagonal (1-D) lin_op only")
+        self._builtin = isinstance(nl_func, BuiltinNL)
+        self._plans = {}
+
+    def set_loglevel(self, loglevel):
+        self.logger.setLevel(_level(loglevel))
+
+    # -- plans -----------------------------------------------------------------
+    def _precision(self, u):
+        if not isinstance(u, torch.Tensor) or not u.is_cuda:
+            raise TypeError("rkstiff_amd solvers step device (HIP) tensors")
+        if u.dtype == torch.complex128:
+            return _lib.F64
+        if u.dtype == torch.complex64:
+            return _lib.F32
+        raise TypeError("state must be complex128 or complex64")
+
+    def _plan(self, u):
+        prec = self._precision(u)
+        key = (prec, u.device.index)
+        p = self._plans.get(key)
+        if p is None:
+            m = self.lin_op.shape[0]
+            if self._builtin:
+                if self.nl_func.modes() != m:
+                    raise ValueError("lin_op and nonlinear term have different numbers of modes")
+                desc = self.nl_func.descriptor(self.lin_op)
+                nl = self.nl_func
+            else:
+                desc = dict(n=m, L=self.lin_op, kx=None, nl_coef=1.0)
+                nl = None
+            p = Plan(self.method, nl, prec, self._config(), self._etd(), [desc],
+                     *self._constants())
+            self._plans[key] = p
+        p.set_config(self._config(), self._etd(), *self._constants())
+        return p
+
+    def _config(self):
+        return None
+
+    def _etd(self):
+        return getattr(self, "etd_config", None)
+
+    def _constants(self):
+        return (50, 4.0, 0.25)
+
+
+class _GenericStages:
+    """Generic-nl path: the method tableau walked from Python, arithmetic on the device."""
+
+    def __init__(self, method):
+        self.t = _lib.tableau(method)
+        self.reset()
+
+    def reset(self):
+        self.h_coeff = None
+        self.started = False
+        self.smap = list(range(_lib.MAX_SLOTS))
+        self.slots = [None] * _lib.MAX_SLOTS
+
+    def _nl(self, nl_func, v, like):
+        out = nl_func(v)
+        if not isinstance(out, torch.Tensor):
+            out = torch.as_tensor(np.asarray(out), device=like.device)
+        return out.to(like.dtype).reshape(like.shape).contiguous()
+
+    def run(self, plan, nl_func, u, h, accept, adaptive):
+        t = self.t
+        u = u.contiguous()
+        if h != self.h_coeff:
+            plan.coeffs(0, h)
+            self.h_coeff = h
+        if not self.started:
+            self.slots[self.smap[0]] = self._nl(nl_func, u, u)
+            self.started = True
+        elif accept or not adaptive:
+            if t.nl1_policy == 1:
+                self.slots[self.smap[0]] = self._nl(nl_func, u, u)
+            else:
+                a = self.smap[0]
+                self.smap[0] = self.smap[t.fsal_slot]
+                self.smap[t.fsal_slot] = a
+        k = None
+        for s in range(t.nstage):
+            S = t.st[s]
+            k = torch.empty_like(u)
+            plan.combine(0, s, h, u, [self.slots[self.smap[j]] for j in range(t.nslot)], k)
+            if S.has_nl:
+                self.slots[self.smap[S.nl_slot]] = self._nl(nl_func, k, u)
+        err = None
+        if adaptive:
+            err = torch.empty_like(u)
+            plan.combine(0, -1, h, None, [self.slots[self.smap[j]] for j in range(t.nslot)], err)
+        return k, err
+
+
+class BaseSolverAS(BaseSolver):
+    """Adaptive-step base: accept/reject controller and evolve (solveras.py:175-650)."""
+
+    class SolverError(RuntimeError):
+        """Base exception for solver-related runtime errors."""
+
+    class MaxLoopsExceeded(SolverError):
+        """Too many attempts to find an acceptable step (solveras.py:399-406)."""
+
+    class MinimumStepReached(SolverError):
+        """Step size fell below config.minh (solveras.py:407-410)."""
+
+    MAX_LOOPS = 50
+    MAX_S = 4.0
+    MIN_S = 0.25
+    SNAP_CHUNK = 128
+
+    def __init__(self, lin_op, nl_func, config=_DEFAULT_CONFIG, loglevel="WARNING"):
+        super().__init__(lin_op, nl_func, loglevel)
+        self.config = config
+        self._accept = False
+        self._generic = None if self._builtin else _GenericStages(self.method)
+        self.attempt_log = []
+
+    @property
+    def solver_type(self):
+        return SolverType.ADAPTIVE_STEP
+
+    def _config(self):
+        return self.config
+
+    def _constants(self):
+        return (self.MAX_LOOPS, self.MAX_S, self.MIN_S)
+
+    def reset(self):
+        self.t, self.u = [], []
+        self._accept = False
+        self._reset()
+
+    def _reset(self):
+        if self._generic is not None:
+            self._generic.reset()
+        for p in self._plans.values():
+            p.reset()
+
+    def _q(self):
+        return _lib.tableau(self.method).q
+
+    # -- step ------------------------------------------------------------------
+    def step(self, u, h_suggest):
+        """One accepted step: (u_new, h_used, h_suggest) (solveras.py:336-410)."""
+        h = float(h_suggest)
+        assert h >= 0.0
+        if self._builtin:
+            return self._step_engine(u, h)
+        return self._step_generic(u, h)
+
+    def _raise_status(self, status):
+        if status == _lib.RKS_MAXLOOPS:
+            msg = ("Solver failed: adaptive step made too many attempts to find a step size "
+                   "with an acceptible amount of error.")
+            self.logger.error(msg)
+            raise self.MaxLoopsExceeded(msg)
+        if status == _lib.RKS_MINSTEP:
+            msg = "Solver failed: adaptive step reached minimum step size"
+            self.logger.error(msg)
+            raise self.MinimumStepReached(msg)
+        if status not in (_lib.RKS_OK, _lib.RKS_PAUSED):
+            raise RuntimeError(f"unexpected solver status {STATUS_NAMES.get(status, status)}")
+
+    def _step_engine(self, u, h):
+        plan = self._plan(u)
+        u = u.contiguous()
+        unew = torch.empty_like(u)
+        plan.step([u], [unew], [h])
+        st = plan.stats()[0]
+        self._raise_status(st["status"])
+        self._accept = True
+        return unew, st["h_used"], st["h_next"]
+
+    def _compute_s(self, plan, unew, err):
+        s_dev = torch.empty(1, dtype=torch.float64, device=unew.device)
+        plan.error_s(0, unew, err, s_dev)
+        return float(s_dev.item())
+
+    def _reject_step_size(self, s, h):
+        """solveras.py:490-506."""
+        self._accept = False
+        if math.isinf(s) or math.isnan(s):
+            self.logger.warning("inf or nan number encountered: reducing step size to %s", h)
+            return self.MIN_S * h
+        s = max(s, self.MIN_S)
+        s = min(s, self.config.decr_f)
+        return s * h
+
+    def _accept_step_size(self, s, h):
+        """solveras.py:543-554."""
+        self._accept = True
+        s = min(s, self.MAX_S)
+        if s > self.config.incr_f:
+            return s * h
+        return h
+
+    def _step_generic(self, u, h):
+        plan = self._plan(u)
+        numloops = 0
+        while True:
+            unew, err = self._generic.run(plan, self.nl_func, u, h, self._accept, True)
+            s = self._compute_s(plan, unew, err)
+            self.attempt_log.append((h, s))
+            if math.isinf(s) or math.isnan(s) or s < 1.0:
+                h = self._reject_step_size(s, h)
+            else:
+                return unew, h, self._accept_step_size(s, h)
+            numloops += 1
+            if numloops > self.MAX_LOOPS:
+                self._raise_status(_lib.RKS_MAXLOOPS)
+            if h < self.config.minh:
+                self._raise_status(_lib.RKS_MINSTEP)
+
+    # -- evolve ----------------------------------------------------------------
+    def evolve(self, u, t0, tf, h_init=None, store_data=True, store_freq=1):
+        """Integrate from t0 to tf (solveras.py:556-650); returns u(tf)."""
+        self.reset()
+        if store_data:
+            self.t.append(t0)
+            self.u.append(u)
+        if h_init is None:
+            h_init = (tf - t0) / 100.0
+        if self._builtin:
+            return self._evolve_engine(u, float(t0), float(tf), float(h_init), store_data,
+                                       int(store_freq))
+        return self._evolve_generic(u, t0, tf, h_init, store_data, store_freq)
+
+    def _evolve_engine(self, u, t0, tf, h, store_data, store_freq):
+        plan = self._plan(u)
+        u = u.contiguous()
+        uf = torch.empty_like(u)
+        plan.set_storage(store_freq if store_data else 0, self.SNAP_CHUNK if store_data else 0,
+                         self._log_capacity())
+        plan.evolve([u], [uf], [t0], [tf], [h])
+        first = True
+        while True:
+            st = plan.stats()[0]
+            if store_data and st["snap_count"] > 0:
+                times, snaps = plan.snapshots(0, st["snap_count"])
+                start = 1 if first else 0  # the kernel stores (t0, u0) too
+                self.t.extend(times[start:])
+                self.u.extend(list(snaps[start:].unbind(0)))
+            first = False
+            if st["status"] == _lib.RKS_PAUSED:
+                plan.resume()
+                continue
+            break
+        self._last_stats = st
+        self._accept = st["steps"] > 0
+        self._raise_status(st["status"])
+        if st["steps"] == 0:
+            return u
+        return uf
+
+    def _log_capacity(self):
+        return getattr(self, "log_capacity", 0)
+
+    def _evolve_generic(self, u, t0, tf, h, store_data, store_freq):
+        tc = t0
+        if tc + h > tf:
+            h = tf - tc
+        step_count = 0
+        while tc < tf:
+            u, h, h_suggest = self.step(u, h)
+            tc += h
+            step_count += 1
+            if tc + h_suggest > tf:
+                h = tf - tc
+            else:
+                h = h_suggest
+            if store_data and (step_count % store_freq == 0):
+                self.t.append(tc)
+                self.u.append(u)
+        return u
+
+    def engine_log(self):
+        """(h, s) of every attempt of the last engine evolve (needs ``log_capacity``)."""
+        st = self._last_stats
+        p = next(iter(self._plans.values()))
+        return p.attempt_log(0, st["log_count"])
+
+
+class BaseSolverCS(BaseSolver):
+    """Constant-step base (solvercs.py:31-279)."""
+
+    SNAP_CHUNK = 128
+
+    def __init__(self, lin_op, nl_func, loglevel="WARNING"):
+        super().__init__(lin_op, nl_func, loglevel)
+        self._generic = None if self._builtin else _GenericStages(self.method)
+
+    @property
+    def solver_type(self):
+        return SolverType.CONSTANT_STEP
+
+    def reset(self):
+        self.t, self.u = [], []
+        self._reset()
+
+    def _reset(self):
+        if self._generic is not None:
+            self._generic.reset()
+        for p in self._plans.values():
+            p.reset()
+
+    def step(self, u, h):
+        """One step of size h (solvercs.py:163-186)."""
+        assert h >= 0.0
+        h = float(h)
+        if self._builtin:
+            plan = self._plan(u)
+            u = u.contiguous()
+            unew = torch.empty_like(u)
+            plan.step([u], [unew], [h])
+            plan.stats()
+            return unew
+        plan = self._plan(u)
+        k, _ = self._generic.run(plan, self.nl_func, u, h, True, False)
+        return k
+
+    def evolve(self, u, t0, tf, h, store_data=True, store_freq=1):
+        """Fixed-step integration from t0 to tf (solvercs.py:191-279)."""
+        self.reset()
+        if store_data:
+            self.t.append(t0)
+            self.u.append(u)
+        if t0 + h > tf:
+            raise ValueError("Step size h must be <= (tf - t0); reduce h or extend tf.")
+        if not self._builtin:
+            tc, step_count = t0, 0
+            while tc < tf:
+                u = self.step(u, h)
+                tc += h
+                step_count += 1
+                if store_data and (step_count % store_freq == 0):
+                    self.t.append(tc)
+                    self.u.append(u)
+            return u
+        plan = self._plan(u)
+        u = u.contiguous()
+        uf = torch.empty_like(u)
+        plan.set_storage(int(store_freq) if store_data else 0, self.SNAP_CHUNK if store_data else 0, 0)
+        plan.evolve([u], [uf], [float(t0)], [float(tf)], [float(h)])
+        first = True
+        while True:
+            st = plan.stats()[0]
+            if store_data and st["snap_count"] > 0:
+                times, snaps = plan.snapshots(0, st["snap_count"])
+                start = 1 if first else 0
+                self.t.extend(times[start:])
+                self.u.extend(list(snaps[start:].unbind(0)))
+            first = False
+            if st["status"] == _lib.RKS_PAUSED:
+                plan.resume()
+                continue
+            break
+        if st["steps"] == 0:
+            return u
+        return uf
+
+
+class ETDAS(BaseSolverAS):
+    def __init__(self, lin_op, nl_func, config=_DEFAULT_CONFIG, etd_config=_DEFAULT_ETD,
+                 loglevel="WARNING"):
+        super().__init__(lin_op, nl_func, config, loglevel)
+        self.etd_config = etd_config
+        self._h_coeff = None
+
+
+class ETDCS(BaseSolverCS):
+    def __init__(self, lin_op, nl_func, etd_config=_DEFAULT_ETD, loglevel="WARNING"):
+        super().__init__(lin_op, nl_func, loglevel)
+        self.etd_config = etd_config
+        self._h_coeff = None
+
+
+class ETD35(ETDAS):
+    """ETD(3,5) adaptive (etd35.py:723-889); q = 4."""
+    method = "ETD35"
+
+    def __init__(self, lin_op, nl_func, config=_DEFAULT_CONFIG, etd_config=_DEFAULT_ETD,
+                 diagonalize=False, loglevel="WARNING"):
+        super().__init__(lin_op, nl_func, config, etd_config, loglevel)
+
+
+class ETD34(ETDAS):
+    """ETD(3,4) adaptive (etd34.py:439-621); q = 4."""
+    method = "ETD34"
+
+    def __init__(self, lin_op, nl_func, config=_DEFAULT_CONFIG, etd_config=_DEFAULT_ETD,
+                 diagonalize=False, loglevel="WARNING"):
+        super().__init__(lin_op, nl_func, config, etd_config, loglevel)
+
+
+class IF34(BaseSolverAS):
+    """IF(3,4) adaptive integrating factor (if34.py:260-338); q = 4."""
+    method = "IF34"
+
+    def __init__(self, lin_op, nl_func, config=_DEFAULT_CONFIG, diagonalize=False,
+                 loglevel="WARNING"):
+        super().__init__(lin_op, nl_func, config, loglevel)
+
+
+class IF45DP(BaseSolverAS):
+    """Dormand-Prince IF 5(4) adaptive (if45dp.py:8-249); q = 5; diagonal only."""
+    method = "IF45DP"
+
+    def __init__(self, lin_op, nl_func, config=_DEFAULT_CONFIG, loglevel="WARNING"):
+        if len(tuple(lin_op.shape)) > 1:
+            raise ValueError("IF45DP only handles 1D linear operators (diagonal systems): "
+                             "try IF34,ETD34, or ETD35")
+        super().__init__(lin_op, nl_func, config, loglevel)
+
+
+class ETD4(ETDCS):
+    """ETD4 (Krogstad) constant step (etd4.py:311-422)."""
+    method = "ETD4"
+
+
+class ETD5(ETDCS):
+    """ETD5 constant step (etd5.py:485-655)."""
+    method = "ETD5"
+
+
+class IF4(BaseSolverCS):
+    """IF4 constant step (if4.py:202-275)."""
+    method = "IF4"

@@ -1,0 +1,145 @@
+"""NumPy interpreter of the device method tableaux (test helper).
+
+Runs a method exactly the way the HIP kernel walks its tableau -- coefficient
+recipes built from basis values, stage terms, NL slot map with FSAL swaps or
+NL1 recomputation -- but in FP64 NumPy, so the tableau transcription of every
+method can be checked against the oracle on the CPU.
+"""
+import numpy as np
+
+from oracle import solvers
+from rkstiff_amd import _lib
+
+
+def basis_values(t, L, h, etd):
+    z = h * L.astype(np.complex128)
+    small = np.zeros(z.shape, bool)
+    if t.uses_contour:
+        small = np.abs(z) < etd.modecutoff
+    nodes = solvers.contour_nodes(etd)
+    out = []
+    for b in range(t.nbasis):
+        bs = t.basis[b]
+        f = bs.fnum / bs.fden
+        if bs.r == 0:
+            out.append(np.exp(bs.fnum * z / bs.fden))
+            continue
+        psi = (solvers.psi1, solvers.psi2, solvers.psi3)[bs.r - 1]
+        v = np.zeros(z.shape, np.complex128)
+        big = ~small
+        with np.errstate(all="ignore"):
+            v[big] = h * psi(f * z[big])
+            if np.any(small):
+                Z = z[small][:, None] + nodes[None, :]
+                v[small] = h * np.mean(psi(f * Z), axis=1)
+        out.append(v)
+    return out
+
+
+def coefficients(t, L, h, etd):
+    B = basis_values(t, L, h, etd)
+    cs = []
+    for c in range(t.ncoef):
+        cf = t.coef[c]
+        acc = 0
+        for k in range(cf.nb):
+            w = cf.w[k] * (h if cf.hmul else 1.0)
+            bv = B[cf.basis[k]] if w == 1.0 else w * B[cf.basis[k]]
+            acc = bv if k == 0 else acc + bv
+        if cf.outer != 1.0:
+            acc = cf.outer * acc
+        cs.append(acc / cf.den if cf.den != 1.0 else acc)
+    return cs
+
+
+class TableauMethod:
+    """Drop-in for an oracle method object, driven by a device tableau."""
+
+    def __init__(self, name, L, nl, etd=None):
+        self.t = _lib.tableau(name)
+        self.name = name
+        self.L = L
+        self.nl = nl
+        self.etd = etd or solvers.ETDConfig()
+        self.adaptive = bool(self.t.adaptive)
+        self.q = self.t.q
+        self.reset()
+
+    def reset(self):
+        self.h_coeff = None
+        self.started = False
+        self.smap = list(range(_lib.MAX_SLOTS))
+        self.slots = [None] * _lib.MAX_SLOTS
+
+    def _term(self, tm, h, C):
+        v = self.slots[self.smap[tm.src]]
+        if tm.src2 >= 0:
+            v = v + tm.scale2 * self.slots[self.smap[tm.src2]]
+        if tm.hmode == 0:
+            c = v if tm.coef < 0 else C[tm.coef] * v
+            return c if tm.scale == 1.0 else tm.scale * c
+        if tm.hmode == 1:
+            c = h * v if tm.coef < 0 else (h * C[tm.coef]) * v
+            return c if tm.scale == 1.0 else tm.scale * c
+        return ((tm.scale * h) / tm.den) * v
+
+    def _stage(self, S, u, h, C):
+        acc = u if S.ecoef < 0 else C[S.ecoef] * u
+        if S.hsum:
+            tot = self._term(S.t[0], h, C)
+            for j in range(1, S.nterm):
+                tot = tot + self._term(S.t[j], h, C)
+            return acc + h * tot
+        for j in range(S.nterm):
+            acc = acc + self._term(S.t[j], h, C)
+        return acc
+
+    def _error(self, h, C):
+        t = self.t
+        ev = self._term(t.err[0], h, C)
+        for j in range(1, t.nerr):
+            ev = ev + self._term(t.err[j], h, C)
+        if t.err_factored:
+            if t.err_hpow:
+                ev = h * ev
+            if t.err_scale != 1.0:
+                ev = t.err_scale * ev
+            if t.err_coef >= 0:
+                ev = C[t.err_coef] * ev
+        return ev
+
+    def _run(self, u, h, accept):
+        t = self.t
+        if h != self.h_coeff:
+            self.C = coefficients(t, self.L, h, self.etd)
+            self.h_coeff = h
+        if not self.started:
+            self.slots[self.smap[0]] = self.nl(u)
+            self.started = True
+        elif accept or not t.adaptive:
+            if t.nl1_policy == 1:
+                self.slots[self.smap[0]] = self.nl(u)
+            else:
+                s0, sf = self.smap[0], self.smap[t.fsal_slot]
+                self.smap[0], self.smap[t.fsal_slot] = sf, s0
+        k = None
+        for s in range(t.nstage):
+            S = t.st[s]
+            k = self._stage(S, u, h, self.C)
+            if S.has_nl:
+                self.slots[self.smap[S.nl_slot]] = self.nl(k)
+        return k
+
+    def attempt(self, u, h, accept):
+        k = self._run(u, h, accept)
+        return k, self._error(h, self.C)
+
+    def advance(self, u, h):
+        return self._run(u, h, True)
+
+
+def make_solver(name, L, nl, config=None, etd=None):
+    m = TableauMethod(name, L, nl, etd)
+    if m.adaptive:
+        return solvers.AdaptiveDriver(m, config)
+    return solvers.ConstantDriver(m)

@@ -1,0 +1,225 @@
+"""GPU parity of the HIP stepping engine against the oracle and the golden fixtures.
+
+Tolerances: FP64 paths are compared at the rounding level their arithmetic
+allows (FFT order and device exp/sin/cos differ from NumPy's by an ulp, which
+the reference's own psi3 cancellation near modecutoff amplifies); FP32 paths
+are compared against FP64 with the tolerance stated per test.
+"""
+import numpy as np
+import pytest
+
+from oracle import grids, problems, solvers
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+ADAPTIVE = ("ETD35", "ETD34", "IF34", "IF45DP")
+CONSTANT = ("ETD4", "ETD5", "IF4")
+
+
+def dev(a, dtype=None):
+    t = torch.as_tensor(np.ascontiguousarray(a))
+    if dtype is not None:
+        t = t.to(dtype)
+    return t.cuda()
+
+
+def host(t):
+    return t.detach().cpu().numpy()
+
+
+def rel(a, b):
+    return solvers.rel_l2(a, b)
+
+
+@pytest.fixture(scope="module")
+def rk():
+    if not torch.cuda.is_available():
+        pytest.skip("no HIP device")
+    import rkstiff_amd
+    from rkstiff_amd import _lib
+    _lib.load()
+    return rkstiff_amd
+
+
+# -- nonlinear terms (FFT round trips) ----------------------------------------
+@pytest.mark.parametrize("n", [16, 32, 64, 128, 256, 512, 1024, 2048, 4096, 8192])
+def test_advective_nl_matches_oracle(rk, n):
+    print(f"advect n={n}", flush=True)
+    x, kx = grids.x_kx_rfft(n, 0.0, 22.0 + n / 64.0)
+    rng = np.random.default_rng(n)
+    u = np.fft.rfft(np.cos(2 * np.pi * x / x[-1]) + 0.3 * rng.normal(size=n))
+    u[n // 4:] *= np.exp(-np.arange(u.size - n // 4) / 8.0)
+    ref = problems.rfft_advect_nl(kx, -1.0)(u)
+    nl = rk.AdvectiveNL(kx, -1.0)
+    got = host(nl(dev(u)))
+    assert rel(got, ref) < 1e-13
+    got32 = host(nl(dev(u, torch.complex64)))
+    assert rel(got32, ref) < 2e-6
+
+
+@pytest.mark.parametrize("n", [16, 64, 256, 1024, 4096, 8192])
+def test_cubic_nl_matches_oracle(rk, n):
+    print(f"cubic n={n}", flush=True)
+    x, kx = grids.x_kx_fft(n, -32.0, 32.0)
+    u = np.fft.fft(1.3 / np.cosh(1.3 * (x - 3)) * np.exp(0.7j * x) + 0.8 / np.cosh(x + 5))
+    ref = problems.nls_nl(1.0)(u)
+    nl = rk.CubicNL(n, 1.0)
+    assert rel(host(nl(dev(u))), ref) < 1e-13
+    assert rel(host(nl(dev(u, torch.complex64))), ref) < 2e-6
+
+
+# -- coefficients ---------------------------------------------------------------
+@pytest.mark.parametrize("method", ["ETD35", "ETD34", "ETD4", "ETD5", "IF34", "IF4", "IF45DP"])
+@pytest.mark.parametrize("lname", ["real", "imag", "cplx"])
+def test_coefficients_match_golden(rk, golden, method, lname):
+    from rkstiff_amd.engine import Plan
+    from tableau_interp import coefficients
+    from rkstiff_amd import _lib
+    g = golden("coeffs")
+    L = g[f"L/{lname}"]
+    t = _lib.tableau(method)
+    plan = Plan(method, None, _lib.F64, None, None, [dict(n=L.size, L=L, kx=None)])
+    worst = 0.0
+    for h in g["h"]:
+        got = host(plan.coeffs(0, float(h)))
+        ref = coefficients(t, L, float(h), solvers.ETDConfig())
+        for c in range(t.ncoef):
+            scale = max(np.max(np.abs(ref[c])), 1e-300)
+            worst = max(worst, float(np.max(np.abs(got[c] - ref[c])) / scale))
+    print(f"{method}/{lname}: max coefficient deviation {worst:.2e}")
+    # direct psi3 near |z| = modecutoff cancels ~1/|z|^3: one ulp of exp is ~1e-10 there
+    assert worst < 5e-9
+
+
+# -- whole trajectories vs golden ----------------------------------------------
+def test_ks1024_if34_fp64_matches_reference(rk, golden):
+    g = golden("ks1024_if34")
+    p = problems.ks_readme()
+    kx = torch.as_tensor(p["kx"])
+    solver = rk.IF34(lin_op=torch.as_tensor(g["L"]), nl_func=rk.AdvectiveNL(kx, -1.0))
+    solver.log_capacity = 4096
+    uf = solver.evolve(dev(g["u0"]), t0=0.0, tf=50.0, store_freq=20)
+    log = solver.engine_log()
+    print("attempts", len(log), "golden", len(g["log_h"]), "rel", rel(host(uf), g["u_final"]))
+    assert len(log) == len(g["log_h"])
+    np.testing.assert_allclose(log[:, 0], g["log_h"], rtol=1e-9, atol=1e-12)
+    assert rel(host(uf), g["u_final"]) < 1e-11
+    np.testing.assert_allclose(solver.t, g["t"], rtol=1e-12)
+    norms = [float(torch.linalg.norm(v)) for v in solver.u]
+    np.testing.assert_allclose(norms, g["snap_norm"], rtol=1e-10)
+
+
+def test_nls8192_etd35_fp64_matches_reference(rk, golden):
+    g = golden("nls8192_etd35")
+    solver = rk.ETD35(lin_op=torch.as_tensor(g["L"]), nl_func=rk.CubicNL(8192, 1.0),
+                      config=rk.SolverConfig(epsilon=1e-6))
+    solver.log_capacity = 4096
+    uf = solver.evolve(dev(g["u0"]), t0=0.0, tf=20.0, store_data=False)
+    log = solver.engine_log()
+    err = rel(host(uf), g["u_final"])
+    print("attempts", len(log), "golden", len(g["log_h"]), "rel", err)
+    assert len(log) == len(g["log_h"])
+    np.testing.assert_allclose(log[:, 0], g["log_h"], rtol=1e-6)
+    assert err < 1e-9
+
+
+@pytest.mark.parametrize("method", ADAPTIVE + CONSTANT)
+def test_kdv_fp64_matches_reference(rk, golden, method):
+    g = golden("kdv_burgers")
+    p = problems.kdv_fixture()
+    nl = rk.AdvectiveNL(torch.as_tensor(p["kx"]), -6.0)
+    L = torch.as_tensor(g["L"])
+    cls = getattr(rk, method)
+    if method in ADAPTIVE:
+        solver = cls(L, nl, config=rk.SolverConfig(epsilon=1e-4))
+        uf = solver.evolve(dev(g["u0"]), t0=0.0, tf=p["tf"], h_init=p["h"], store_data=True,
+                           store_freq=10)
+    else:
+        solver = cls(L, nl)
+        uf = solver.evolve(dev(g["u0"]), t0=0.0, tf=p["tf"], h=p["h"], store_data=True,
+                           store_freq=10)
+    err = rel(host(uf), g[f"{method}/u_final"])
+    print(method, "rel", err)
+    assert err < 1e-11
+    np.testing.assert_allclose(solver.t, g[f"{method}/t"], rtol=1e-12)
+    # the reference's own accuracy bar vs the analytic soliton (test_etd4.py:19 etc.)
+    bar = {"ETD4": 1e-6, "ETD5": 1e-6, "IF4": 1e-5}.get(method, 1e-4)
+    assert rel(host(uf), p["exact"]) < bar
+
+
+@pytest.mark.parametrize("pname", ["ks64", "nls64"])
+@pytest.mark.parametrize("method", ADAPTIVE + CONSTANT)
+def test_three_steps_fp64(rk, golden, pname, method):
+    g = golden("attempts")
+    L, u = g[f"{pname}/L"], g[f"{pname}/u0"]
+    if pname == "ks64":
+        _, kx = grids.x_kx_rfft(64, 0.0, 32.0 * np.pi)
+        nl, h0 = rk.AdvectiveNL(torch.as_tensor(kx), -1.0), 0.05
+    else:
+        nl, h0 = rk.CubicNL(64, 1.0), 0.02
+    cls = getattr(rk, method)
+    if method in ADAPTIVE:
+        solver = cls(torch.as_tensor(L), nl, config=rk.SolverConfig(epsilon=1e-3))
+    else:
+        solver = cls(torch.as_tensor(L), nl)
+    v, h = dev(u), h0
+    for i in range(3):
+        if method in ADAPTIVE:
+            v, hu, h = solver.step(v, h)
+            np.testing.assert_allclose(hu, g[f"{pname}/{method}/hused{i + 1}"], rtol=1e-10)
+            np.testing.assert_allclose(h, g[f"{pname}/{method}/hnext{i + 1}"], rtol=1e-10)
+        else:
+            v = solver.step(v, h0)
+        assert rel(host(v), g[f"{pname}/{method}/u{i + 1}"]) < 1e-12
+
+
+@pytest.mark.parametrize("method", ADAPTIVE + CONSTANT)
+def test_generic_path_matches_reference(rk, golden, method):
+    """Arbitrary torch nl_func: stage combinations and error norm through the C ABI."""
+    g = golden("nls256")
+    L, u0 = g["L"], g["u0"]
+
+    def nl(u):
+        v = torch.fft.ifft(u)
+        return 1j * torch.fft.fft((v.real**2 + v.imag**2) * v)
+    cls = getattr(rk, method)
+    if method in ADAPTIVE:
+        solver = cls(torch.as_tensor(L), nl, config=rk.SolverConfig(epsilon=1e-5))
+        uf = solver.evolve(dev(u0), t0=0.0, tf=2.0, store_data=False)
+    else:
+        solver = cls(torch.as_tensor(L), nl)
+        uf = solver.evolve(dev(u0), t0=0.0, tf=2.0, h=0.01, store_data=False)
+    err = rel(host(uf), g[f"{method}/u_final"])
+    print(method, "generic rel", err)
+    assert err < 1e-10
+
+
+@pytest.mark.parametrize("method", ADAPTIVE + CONSTANT)
+def test_nls256_engine_fp64(rk, golden, method):
+    g = golden("nls256")
+    L, u0 = g["L"], g["u0"]
+    cls = getattr(rk, method)
+    nl = rk.CubicNL(256, 1.0)
+    if method in ADAPTIVE:
+        solver = cls(torch.as_tensor(L), nl, config=rk.SolverConfig(epsilon=1e-5))
+        uf = solver.evolve(dev(u0), t0=0.0, tf=2.0, store_data=False)
+    else:
+        solver = cls(torch.as_tensor(L), nl)
+        uf = solver.evolve(dev(u0), t0=0.0, tf=2.0, h=0.01, store_data=False)
+    err = rel(host(uf), g[f"{method}/u_final"])
+    print(method, "engine rel", err)
+    assert err < 1e-10
+
+
+@pytest.mark.parametrize("method", ["ETD35", "IF34"])
+def test_kdv_fp32_close_to_fp64(rk, golden, method):
+    g = golden("kdv_burgers")
+    p = problems.kdv_fixture()
+    nl = rk.AdvectiveNL(torch.as_tensor(p["kx"]), -6.0)
+    solver = getattr(rk, method)(torch.as_tensor(g["L"]), nl, config=rk.SolverConfig(epsilon=1e-4))
+    uf = solver.evolve(dev(g["u0"], torch.complex64), t0=0.0, tf=p["tf"], h_init=p["h"],
+                       store_data=False)
+    err = rel(host(uf).astype(np.complex128), g[f"{method}/u_final"])
+    print(method, "fp32 vs fp64 rel", err)
+    assert err < 1e-4

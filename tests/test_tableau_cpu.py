@@ -1,0 +1,92 @@
+"""Device tableaux (rks_tableau.cpp, read through the C ABI without a GPU) encode
+the reference methods: a NumPy interpreter of the tableau matches the oracle.
+
+Also checks that the built library exports every symbol include/rkstiff_amd.h declares.
+"""
+import os
+import re
+
+import numpy as np
+import pytest
+
+from oracle import grids, problems, solvers
+from rkstiff_amd import _lib
+
+from tableau_interp import coefficients, make_solver
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ADAPTIVE = ("ETD35", "ETD34", "IF34", "IF45DP")
+CONSTANT = ("ETD4", "ETD5", "IF4")
+
+ORACLE_COEFFS = {
+    "ETD35": solvers.etd35_coeffs, "ETD34": solvers.etd34_coeffs, "ETD4": solvers.etd4_coeffs,
+    "ETD5": solvers.etd5_coeffs,
+}
+
+
+def test_library_exports_header_symbols():
+    lib = _lib.load()
+    hdr = open(os.path.join(ROOT, "include", "rkstiff_amd.h")).read()
+    names = set(re.findall(r"^\s*(?:int|const char\*)\s+(rks_\w+)\(", hdr, re.M))
+    assert len(names) >= 18
+    for name in names:
+        assert hasattr(lib, name), name
+    assert set(_lib.EXPORTS) <= names | {"rks_set_config"}
+
+
+@pytest.mark.parametrize("method", list(ORACLE_COEFFS))
+@pytest.mark.parametrize("lname", ["real", "imag", "cplx"])
+def test_tableau_coefficients_match_oracle(golden, method, lname):
+    g = golden("coeffs")
+    L = g[f"L/{lname}"]
+    t = _lib.tableau(method)
+    etd = solvers.ETDConfig()
+    for h in g["h"]:
+        ref = ORACLE_COEFFS[method](L, float(h), etd)
+        got = coefficients(t, L, float(h), etd)
+        # every oracle array appears among the tableau's coefficient arrays
+        for name, arr in ref.items():
+            if name == "a64":
+                continue
+            errs = [np.max(np.abs(c - arr)) / max(1e-300, np.max(np.abs(arr))) for c in got]
+            assert min(errs) < 1e-12, (name, min(errs))
+
+
+@pytest.mark.parametrize("method", ADAPTIVE + CONSTANT)
+def test_tableau_kdv_matches_golden(golden, method):
+    g = golden("kdv_burgers")
+    p = problems.kdv_fixture()
+    if method in ADAPTIVE:
+        drv = make_solver(method, p["L"], p["nl"], solvers.SolverConfig(epsilon=1e-4))
+        uf = drv.evolve(p["u0"], t0=0.0, tf=p["tf"], h_init=p["h"], store_data=True, store_freq=10)
+        assert len(drv.log) == len(g[f"{method}/log_h"])
+        # same grouping as the reference: h sequence agrees to rounding (the clamped
+        # last step tf - tc is a difference of nearly equal times, hence atol)
+        np.testing.assert_allclose([e[0] for e in drv.log], g[f"{method}/log_h"], rtol=1e-12,
+                                   atol=1e-14)
+    else:
+        drv = make_solver(method, p["L"], p["nl"])
+        uf = drv.evolve(p["u0"], t0=0.0, tf=p["tf"], h=p["h"], store_data=True, store_freq=10)
+    assert solvers.rel_l2(uf, g[f"{method}/u_final"]) < 1e-12
+    np.testing.assert_allclose(drv.t, g[f"{method}/t"], rtol=1e-13)
+
+
+@pytest.mark.parametrize("pname", ["ks64", "nls64"])
+@pytest.mark.parametrize("method", ADAPTIVE + CONSTANT)
+def test_tableau_three_steps(golden, pname, method):
+    g = golden("attempts")
+    L, u = g[f"{pname}/L"], g[f"{pname}/u0"]
+    if pname == "ks64":
+        _, kx = grids.x_kx_rfft(64, 0.0, 32.0 * np.pi)
+        nl, h0 = problems.rfft_advect_nl(kx, -1.0), 0.05
+    else:
+        nl, h0 = problems.nls_nl(1.0), 0.02
+    drv = make_solver(method, L, nl, solvers.SolverConfig(epsilon=1e-3))
+    v, h = u.copy(), h0
+    for i in range(3):
+        if method in ADAPTIVE:
+            v, hu, h = drv.step(v, h)
+            np.testing.assert_allclose(hu, g[f"{pname}/{method}/hused{i + 1}"], rtol=1e-12)
+        else:
+            v = drv.step(v, h0)
+        assert solvers.rel_l2(v, g[f"{pname}/{method}/u{i + 1}"]) < 1e-13
