@@ -48,7 +48,7 @@ class Stats(ctypes.Structure):
 
 
 # tableau mirror (rkstiff_amd/csrc/rks_types.h)
-MAX_STAGES, MAX_TERMS, MAX_COEF, MAX_BASIS, MAX_SLOTS = 8, 8, 32, 16, 8
+MAX_STAGES, MAX_TERMS, MAX_COEF, MAX_FRAC, MAX_SLOTS = 8, 8, 32, 16, 8
 
 
 class Term(ctypes.Structure):
@@ -63,25 +63,26 @@ class Stage(ctypes.Structure):
                 ("t", Term * MAX_TERMS)]
 
 
-class Basis(ctypes.Structure):
-    _fields_ = [("r", ctypes.c_int), ("fnum", ctypes.c_int), ("fden", ctypes.c_int),
+class Frac(ctypes.Structure):
+    _fields_ = [("fnum", ctypes.c_int), ("fden", ctypes.c_int), ("mask", ctypes.c_int),
                 ("pad", ctypes.c_int)]
 
 
 class Coef(ctypes.Structure):
-    _fields_ = [("nb", ctypes.c_int), ("hmul", ctypes.c_int), ("basis", ctypes.c_int * 4),
-                ("w", ctypes.c_double * 4), ("outer", ctypes.c_double), ("den", ctypes.c_double)]
+    _fields_ = [("nb", ctypes.c_int), ("hmul", ctypes.c_int), ("frac", ctypes.c_int),
+                ("pad", ctypes.c_int), ("kind", ctypes.c_int * 4), ("w", ctypes.c_double * 4),
+                ("outer", ctypes.c_double), ("den", ctypes.c_double)]
 
 
 class Tableau(ctypes.Structure):
     _fields_ = [("method", ctypes.c_int), ("adaptive", ctypes.c_int), ("q", ctypes.c_int),
                 ("nl1_policy", ctypes.c_int), ("fsal_slot", ctypes.c_int), ("nslot", ctypes.c_int),
                 ("nstage", ctypes.c_int), ("nerr", ctypes.c_int), ("ncoef", ctypes.c_int),
-                ("nbasis", ctypes.c_int), ("uses_contour", ctypes.c_int), ("err_coef", ctypes.c_int),
+                ("nfrac", ctypes.c_int), ("uses_contour", ctypes.c_int), ("err_coef", ctypes.c_int),
                 ("err_factored", ctypes.c_int), ("err_hpow", ctypes.c_int), ("pad2", ctypes.c_int),
                 ("err_scale", ctypes.c_double),
                 ("st", Stage * MAX_STAGES), ("err", Term * MAX_TERMS),
-                ("coef", Coef * MAX_COEF), ("basis", Basis * MAX_BASIS)]
+                ("coef", Coef * MAX_COEF), ("frac", Frac * MAX_FRAC)]
 
 
 _lib = None

@@ -8,35 +8,77 @@
 //   read  buf[j + r*P/RAD]                          (r < RAD, j < P/RAD)
 //   twiddle by W_{NS*RAD}^{r*(j mod NS)}
 //   write buf[(j/NS)*NS*RAD + (j mod NS) + r*NS]
-// Radices are the largest power of two <= 16 that divides the per-thread
-// element count E = P/T, chosen at compile time.  Twiddles come from a table
-// tw[x] = exp(-2 pi i x / TN) of the largest transform size TN in use; a
-// P-point pass reads it with stride TN/P.  INV selects exp(+...) (unnormalised).
+// Layout and twiddles:
+//   * element x lives at LDS index pad(x) = x + x/16: one spare element every
+//     16 makes the stride-RAD writes of the first pass and all later passes
+//     free of bank conflicts for 8- and 16-byte elements;
+//   * W_N^x = T1[x >> S] * T2[x & (2^S-1)] from two small LDS tables
+//     (sqrt(N) entries each, filled once per workgroup from the correctly
+//     rounded global table), so no pass waits on global memory;
+//   * radix = largest power of two <= 32 dividing the per-thread element
+//     count E = P/T (radix-32 first: 8192 points with E = 32 take 3 passes).
+// INV selects exp(+...) (unnormalised).
 #pragma once
 #include "rks_math.h"
 
 namespace rksfft {
 
-// exp(-2 pi i k / 16), k = 0..7, correctly rounded
-__device__ __forceinline__ double c16(int k) {
-  const double v[8] = {1.0, 0.92387953251128675613, 0.70710678118654752440, 0.38268343236508977173,
-                       0.0, -0.38268343236508977173, -0.70710678118654752440, -0.92387953251128675613};
+__device__ __forceinline__ constexpr int pad(int x) { return x + (x >> 4); }
+// LDS elements needed for a padded P-point buffer
+__host__ __device__ constexpr int padded_size(int P) { return P + (P >> 4) + 1; }
+
+// two-level twiddle table geometry for an N-point base table
+template <int LOG2N>
+struct TwSplit {
+  static constexpr int S = (LOG2N + 1) / 2;
+  static constexpr int N1 = (1 << LOG2N) >> S;  // T1[i] = W_N^(i << S)
+  static constexpr int N2 = 1 << S;              // T2[j] = W_N^j
+  static constexpr int SIZE = N1 + N2;
+};
+
+// fill the LDS tables from the global table tw[x] = exp(-2 pi i x / N)
+template <typename R, int LOG2N, int T>
+__device__ __forceinline__ void load_twiddles(cx<R>* twl, const cx<R>* __restrict__ tw) {
+  using TS = TwSplit<LOG2N>;
+  for (int i = threadIdx.x; i < TS::SIZE; i += T)
+    twl[i] = (i < TS::N1) ? tw[i << TS::S] : tw[i - TS::N1];
+}
+
+// W_N^x, 0 <= x < N
+template <typename R, int LOG2N>
+__device__ __forceinline__ cx<R> twiddle(const cx<R>* twl, int x) {
+  using TS = TwSplit<LOG2N>;
+  const cx<R> a = twl[x >> TS::S];
+  const cx<R> b = twl[TS::N1 + (x & (TS::N2 - 1))];
+  return a * b;
+}
+
+// exp(-2 pi i k / 32), k = 0..15, correctly rounded
+__device__ __forceinline__ double c32(int k) {
+  const double v[16] = {1.0, 0.9807852804032304491262, 0.9238795325112867561282, 0.8314696123025452370788,
+                        0.7071067811865475244008, 0.5555702330196022247428, 0.3826834323650897717285,
+                        0.1950903220161282678483, 0.0, -0.1950903220161282678483, -0.3826834323650897717285,
+                        -0.5555702330196022247428, -0.7071067811865475244008, -0.8314696123025452370788,
+                        -0.9238795325112867561282, -0.9807852804032304491262};
   return v[k];
 }
-__device__ __forceinline__ double s16(int k) {
-  const double v[8] = {0.0, 0.38268343236508977173, 0.70710678118654752440, 0.92387953251128675613,
-                       1.0, 0.92387953251128675613, 0.70710678118654752440, 0.38268343236508977173};
+__device__ __forceinline__ double s32(int k) {
+  const double v[16] = {0.0, 0.1950903220161282678483, 0.3826834323650897717285, 0.5555702330196022247428,
+                        0.7071067811865475244008, 0.8314696123025452370788, 0.9238795325112867561282,
+                        0.9807852804032304491262, 1.0, 0.9807852804032304491262, 0.9238795325112867561282,
+                        0.8314696123025452370788, 0.7071067811865475244008, 0.5555702330196022247428,
+                        0.3826834323650897717285, 0.1950903220161282678483};
   return v[k];
 }
 
 // d * W_len^i (forward: exp(-2 pi i i/len); inverse: conjugate)
 template <int LEN, bool INV, typename R>
 __device__ __forceinline__ cx<R> tw_const(int i, cx<R> d) {
-  const int k = i * (16 / LEN);
+  const int k = i * (32 / LEN);
   if (k == 0) return d;
-  if (k == 4) return mul_mi<INV>(d);
-  const R c = (R)c16(k);
-  const R s = INV ? (R)s16(k) : (R)(-s16(k));
+  if (k == 8) return mul_mi<INV>(d);
+  const R c = (R)c32(k);
+  const R s = INV ? (R)s32(k) : (R)(-s32(k));
   return mkc(d.re * c - d.im * s, d.re * s + d.im * c);
 }
 
@@ -63,7 +105,8 @@ __device__ __forceinline__ void dft_small(cx<R> (&v)[RAD]) {
         cx<R> a = v[blk + i], b = v[blk + i + half];
         v[blk + i] = a + b;
         cx<R> d = a - b;
-        if (len == 16) v[blk + i + half] = tw_const<16, INV>(i, d);
+        if (len == 32) v[blk + i + half] = tw_const<32, INV>(i, d);
+        else if (len == 16) v[blk + i + half] = tw_const<16, INV>(i, d);
         else if (len == 8) v[blk + i + half] = tw_const<8, INV>(i, d);
         else if (len == 4) v[blk + i + half] = tw_const<4, INV>(i, d);
         else v[blk + i + half] = d;
@@ -77,13 +120,18 @@ __device__ __forceinline__ void dft_small(cx<R> (&v)[RAD]) {
   for (int i = 0; i < RAD; ++i) v[i] = t[i];
 }
 
-template <int E>
+template <int E, int MAXR>
 struct pick_radix {
-  static constexpr int v = (E % 16 == 0) ? 16 : (E % 8 == 0) ? 8 : (E % 4 == 0) ? 4 : 2;
+  static constexpr int v = (E % 32 == 0 && MAXR >= 32) ? 32
+                           : (E % 16 == 0 && MAXR >= 16) ? 16
+                           : (E % 8 == 0 && MAXR >= 8) ? 8
+                           : (E % 4 == 0 && MAXR >= 4) ? 4 : 2;
 };
 
-template <typename R, int P, int T, int NS, int RAD, bool INV>
-__device__ __forceinline__ void pass(cx<R>* buf, const cx<R>* __restrict__ tw, int tws) {
+// One Stockham pass.  Twiddle W_{NS*RAD}^{r k} = W_N^{r k (P/(NS RAD)) TSCALE}, where
+// TSCALE = N / P maps a P-point transform onto the N-point table.
+template <typename R, int P, int T, int NS, int RAD, bool INV, int LOG2N, int TSCALE>
+__device__ __forceinline__ void pass(cx<R>* buf, const cx<R>* twl) {
   constexpr int NB = P / (RAD * T);
   static_assert(NB >= 1, "too few elements per thread");
   const int t = threadIdx.x;
@@ -92,7 +140,7 @@ __device__ __forceinline__ void pass(cx<R>* buf, const cx<R>* __restrict__ tw, i
   for (int b = 0; b < NB; ++b) {
     const int j = t + b * T;
 #pragma unroll
-    for (int r = 0; r < RAD; ++r) v[b][r] = buf[j + r * (P / RAD)];
+    for (int r = 0; r < RAD; ++r) v[b][r] = buf[pad(j + r * (P / RAD))];
   }
   __syncthreads();
 #pragma unroll
@@ -100,37 +148,37 @@ __device__ __forceinline__ void pass(cx<R>* buf, const cx<R>* __restrict__ tw, i
     const int j = t + b * T;
     const int k = j & (NS - 1);
     if (NS > 1) {
-      const int step = k * (P / (NS * RAD)) * tws;
+      const int step = k * (P / (NS * RAD)) * TSCALE;
 #pragma unroll
-      for (int r = 1; r < RAD; ++r) v[b][r] = cmul_tw<INV>(v[b][r], tw[r * step]);
+      for (int r = 1; r < RAD; ++r) v[b][r] = cmul_tw<INV>(v[b][r], twiddle<R, LOG2N>(twl, r * step));
     }
     dft_small<RAD, INV>(v[b]);
     const int base = (j - k) * RAD + k;
 #pragma unroll
-    for (int r = 0; r < RAD; ++r) buf[base + r * NS] = v[b][r];
+    for (int r = 0; r < RAD; ++r) buf[pad(base + r * NS)] = v[b][r];
   }
   __syncthreads();
 }
 
-template <typename R, int P, int T, int NS, bool INV>
-__device__ __forceinline__ void chain(cx<R>* buf, const cx<R>* __restrict__ tw, int tws) {
+template <typename R, int P, int T, int NS, bool INV, int LOG2N, int TSCALE, int MAXR>
+__device__ __forceinline__ void chain(cx<R>* buf, const cx<R>* twl) {
   if constexpr (NS < P) {
     constexpr int E = P / T;
-    constexpr int R0 = pick_radix<E>::v;
+    constexpr int R0 = pick_radix<E, MAXR>::v;
     constexpr int REM = P / NS;
     constexpr int RAD = (R0 <= REM) ? R0 : REM;
-    pass<R, P, T, NS, RAD, INV>(buf, tw, tws);
-    chain<R, P, T, NS * RAD, INV>(buf, tw, tws);
+    pass<R, P, T, NS, RAD, INV, LOG2N, TSCALE>(buf, twl);
+    chain<R, P, T, NS * RAD, INV, LOG2N, TSCALE, MAXR>(buf, twl);
   }
 }
 
-// Full P-point transform in LDS.  Entry and exit are workgroup-synchronised
-// by the caller/pass barriers.
-template <typename R, int P, int T, bool INV>
-__device__ __forceinline__ void fft(cx<R>* buf, const cx<R>* __restrict__ tw, int tws) {
+// Full P-point transform in LDS (padded layout).  twl: LDS tables of the
+// 2^LOG2N-point base table (P divides 2^LOG2N).  Callers synchronise before.
+template <typename R, int P, int T, bool INV, int LOG2N, int MAXR = 32>
+__device__ __forceinline__ void fft(cx<R>* buf, const cx<R>* twl) {
   static_assert((P & (P - 1)) == 0, "power of two");
   static_assert(P / T >= 2, "need at least two elements per thread");
-  chain<R, P, T, 1, INV>(buf, tw, tws);
+  chain<R, P, T, 1, INV, LOG2N, (1 << LOG2N) / P, MAXR>(buf, twl);
 }
 
 }  // namespace rksfft

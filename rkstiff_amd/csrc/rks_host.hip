@@ -134,7 +134,7 @@ extern template __global__ void rks_kernel<float, 13, NLK_CUBIC>(rks_launch);
 // ===========================================================================
 // host side: plans and the C ABI
 // ===========================================================================
-#include "../../include/rkstiff_amd.h"
+#include "rkstiff_amd.h"
 
 int rks_build_tableau(int method, rks_tableau* out);
 
@@ -166,7 +166,7 @@ KernelEntry entry_of() {
   KernelEntry e;
   e.fn = (const void*)&rks_kernel<R, LOG2N, NLK>;
   e.threads = rks_threads(LOG2N, NLK);
-  e.lds = (1 << LOG2N) * (int)sizeof(cx<R>);
+  e.lds = rks_lds_bytes<R, LOG2N>();
   return e;
 }
 

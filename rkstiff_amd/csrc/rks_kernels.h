@@ -41,12 +41,22 @@ struct rks_launch {
   double h_mode;   // RKS_MODE_COEFFS
 };
 
-// threads per workgroup for an N-point problem
+// threads per workgroup for an N-point problem (N/16 elements... capped at 512,
+// at least two elements per thread in every transform)
 __host__ __device__ constexpr int rks_threads(int log2n, int nlk) {
   const int N = 1 << log2n;
   const int lo = (nlk == NLK_ADVECT) ? (N / 4 < 64 ? N / 4 : 64) : (N / 2 < 64 ? N / 2 : 64);
   const int t = N / 16 > lo ? N / 16 : lo;
-  return t > 256 ? 256 : t;
+  return t > 512 ? 512 : t;
+}
+
+// largest in-register radix of the LDS FFT passes
+#define RKS_MAXR 16
+
+// LDS bytes of the persistent kernel: padded FFT buffer + twiddle tables
+template <typename R, int LOG2N>
+__host__ __device__ constexpr int rks_lds_bytes() {
+  return (rksfft::padded_size(1 << LOG2N) + rksfft::TwSplit<LOG2N>::SIZE) * (int)sizeof(cx<R>);
 }
 
 __device__ __forceinline__ double nanmax(double a, double b) {
@@ -107,57 +117,55 @@ __device__ __forceinline__ z64 frac_arg(z64 z, int num, int den, int cplx) {
   return mkc(w.re / (double)den, w.im / (double)den);
 }
 
-template <typename R>
-__device__ void coef_mode(const rks_tableau& tb, const rks_ctrl& ct, const double* __restrict__ L,
-                          int L_complex, int m, int M, double h, cx<R>* __restrict__ coefs) {
+__device__ __forceinline__ z64 contour_node(const rks_ctrl& ct, int j) {
+  // R exp(2 pi i (j + 1/2) / N)   (etd35.py:259)
+  const double th = (6.283185307179586 * (j + 0.5)) * (1.0 / (double)ct.contour_points);
+  double sn, cs;
+  sincos(th, &sn, &cs);
+  return mkc(ct.contour_radius * cs, ct.contour_radius * sn);
+}
+
+__device__ __forceinline__ z64 mode_z(const double* __restrict__ L, int L_complex, int m, double h) {
+  const z64 Lm = L_complex ? mkc(L[2 * m], L[2 * m + 1]) : mkc(L[m], 0.0);
+  return mkc(h * Lm.re, h * Lm.im);
+}
+
+// Basis values (exp, h psi1, h psi2, h psi3) of fraction f by the closed forms.
+__device__ __forceinline__ void frac_direct(const rks_frac& fr, z64 z, double h, int cplx, z64 (&v)[4]) {
 #pragma clang fp contract(off)
-  // ETD methods cast lin_op to complex128 (etd35.py:124); IF methods keep its dtype
-  const int cplx = L_complex || tb.uses_contour;
-  z64 Lm = L_complex ? mkc(L[2 * m], L[2 * m + 1]) : mkc(L[m], 0.0);
-  z64 z = mkc(h * Lm.re, h * Lm.im);
-  const bool small = tb.uses_contour && (hypot(z.re, z.im) < ct.modecutoff);
-  z64 B[RKS_MAX_BASIS];
-#pragma unroll
-  for (int b = 0; b < RKS_MAX_BASIS; ++b) {
-    if (b >= tb.nbasis) break;
-    const rks_basis bs = tb.basis[b];
-    if (bs.r == RKS_BASIS_EXP) {
-      B[b] = np_exp(frac_arg(z, bs.fnum, bs.fden, cplx));
-    } else if (!small) {
-      z64 p1, p2, p3;
-      np_psi123(frac_arg(z, bs.fnum, bs.fden, 1), bs.r == RKS_BASIS_PSI3, &p1, &p2, &p3);
-      const z64 p = bs.r == RKS_BASIS_PSI1 ? p1 : (bs.r == RKS_BASIS_PSI2 ? p2 : p3);
-      B[b] = np_rmul(h, p);
-    } else {
-      // contour mean over nodes R exp(2 pi i (j + 1/2) / N)   (etd35.py:255-264)
-      const int np_ = ct.contour_points;
-      const double inv_np = 1.0 / (double)np_;
-      z64 acc[4] = {mkc(0.0, 0.0), mkc(0.0, 0.0), mkc(0.0, 0.0), mkc(0.0, 0.0)};
-      for (int j = 0; j < np_; ++j) {
-        const double th = (6.283185307179586 * (j + 0.5)) * inv_np;
-        double sn, cs;
-        sincos(th, &sn, &cs);
-        const z64 node = mkc(ct.contour_radius * cs, ct.contour_radius * sn);
-        const z64 Z = mkc(z.re + node.re, z.im + node.im);
-        z64 p1, p2, p3;
-        np_psi123(frac_arg(Z, bs.fnum, bs.fden, 1), bs.r == RKS_BASIS_PSI3, &p1, &p2, &p3);
-        const z64 p = bs.r == RKS_BASIS_PSI1 ? p1 : (bs.r == RKS_BASIS_PSI2 ? p2 : p3);
-        acc[j & 3] = mkc(acc[j & 3].re + p.re, acc[j & 3].im + p.im);
-      }
-      const z64 s = mkc((acc[0].re + acc[1].re) + (acc[2].re + acc[3].re),
-                        (acc[0].im + acc[1].im) + (acc[2].im + acc[3].im));
-      B[b] = np_rdiv(np_rmul(h, s), (double)np_);
-    }
+  const z64 e = np_exp(frac_arg(z, fr.fnum, fr.fden, cplx));
+  v[0] = e;
+  if (fr.mask & 14) {
+    z64 p1, p2, p3 = mkc(0.0, 0.0);
+    np_psi123_e(frac_arg(z, fr.fnum, fr.fden, 1), e, fr.mask & 8, &p1, &p2, &p3);
+    v[1] = np_rmul(h, p1);
+    v[2] = np_rmul(h, p2);
+    v[3] = np_rmul(h, p3);
   }
+}
+
+// psi_r(f Z) summed into acc[1..3] at one contour point Z.
+__device__ __forceinline__ void frac_contour_add(const rks_frac& fr, z64 Z, z64 (&acc)[4]) {
+#pragma clang fp contract(off)
+  z64 p1, p2, p3 = mkc(0.0, 0.0);
+  np_psi123(frac_arg(Z, fr.fnum, fr.fden, 1), fr.mask & 8, &p1, &p2, &p3);
+  acc[1] = mkc(acc[1].re + p1.re, acc[1].im + p1.im);
+  acc[2] = mkc(acc[2].re + p2.re, acc[2].im + p2.im);
+  acc[3] = mkc(acc[3].re + p3.re, acc[3].im + p3.im);
+}
+
+// Every coefficient recipe of fraction f: c = (outer * sum_k w_k B_k) / den.
+template <typename R>
+__device__ __forceinline__ void frac_coefs(const rks_tableau& tb, int f, const z64 (&v)[4], double h, int cplx,
+                                           int m, size_t M, cx<R>* __restrict__ coefs) {
+#pragma clang fp contract(off)
   for (int c = 0; c < tb.ncoef; ++c) {
-    const rks_coef cf = tb.coef[c];
+    const rks_coef& cf = tb.coef[c];
+    if (cf.frac != f) continue;
     z64 acc = mkc(0.0, 0.0);
     for (int k = 0; k < cf.nb; ++k) {
-      int bi = cf.basis[k];
-      z64 bv = B[0];
-#pragma unroll
-      for (int b = 1; b < RKS_MAX_BASIS; ++b)
-        if (b == bi) bv = B[b];
+      const int kd = cf.kind[k];
+      const z64 bv = kd == 0 ? v[0] : (kd == 1 ? v[1] : (kd == 2 ? v[2] : v[3]));
       const double w = cf.hmul ? cf.w[k] * h : cf.w[k];
       const z64 t = (w == 1.0) ? bv : np_rmul(w, bv);
       acc = (k == 0) ? t : mkc(acc.re + t.re, acc.im + t.im);
@@ -169,6 +177,110 @@ __device__ void coef_mode(const rks_tableau& tb, const rks_ctrl& ct, const doubl
     }
     coefs[(size_t)c * M + m] = mkc((R)acc.re, (R)acc.im);
   }
+}
+
+// All coefficients of one mode by one thread (generic-path kernel).
+template <typename R>
+__device__ void coef_mode(const rks_tableau& tb, const rks_ctrl& ct, const double* __restrict__ L,
+                          int L_complex, int m, int M, double h, cx<R>* __restrict__ coefs) {
+#pragma clang fp contract(off)
+  const int cplx = L_complex || tb.uses_contour;
+  const z64 z = mode_z(L, L_complex, m, h);
+  const bool small = tb.uses_contour && hypot(z.re, z.im) < ct.modecutoff;
+  for (int f = 0; f < tb.nfrac; ++f) {
+    const rks_frac fr = tb.frac[f];
+    z64 v[4];
+    v[1] = v[2] = v[3] = mkc(0.0, 0.0);
+    frac_direct(fr, z, h, cplx, v);
+    if (small && (fr.mask & 14)) {
+      z64 acc[4];
+      acc[1] = acc[2] = acc[3] = mkc(0.0, 0.0);
+      for (int j = 0; j < ct.contour_points; ++j) {
+        const z64 nd = contour_node(ct, j);
+        frac_contour_add(fr, mkc(z.re + nd.re, z.im + nd.im), acc);
+      }
+      for (int r = 1; r <= 3; ++r) {
+        const z64 a = r == 1 ? acc[1] : (r == 2 ? acc[2] : acc[3]);
+        const z64 b = np_rdiv(np_rmul(h, a), (double)ct.contour_points);
+        if (r == 1) v[1] = b;
+        else if (r == 2) v[2] = b;
+        else v[3] = b;
+      }
+    }
+    frac_coefs<R>(tb, f, v, h, cplx, m, (size_t)M, coefs);
+  }
+}
+
+// All coefficients of an instance by one workgroup: closed forms per thread,
+// then the small modes' contour means spread over groups of up to 32 lanes
+// (one lane per node, deterministic shuffle reduction).  `list` is LDS
+// scratch for up to M ints; *count is an LDS counter.
+template <typename R, int T, int MPT>
+__device__ __attribute__((noinline)) void coef_update(const rks_tableau& tb, const rks_ctrl& ct, const double* __restrict__ L,
+                            int L_complex, int M, double h, cx<R>* __restrict__ coefs, int* list, int* count) {
+#pragma clang fp contract(off)
+  const int t = threadIdx.x;
+  const int cplx = L_complex || tb.uses_contour;
+  if (t == 0) *count = 0;
+  __syncthreads();
+  for (int i = 0; i < MPT; ++i) {
+    const int m = t + i * T;
+    if (m >= M) break;
+    const z64 z = mode_z(L, L_complex, m, h);
+    if (tb.uses_contour && hypot(z.re, z.im) < ct.modecutoff) {
+      list[atomicAdd(count, 1)] = m;
+      continue;
+    }
+    for (int f = 0; f < tb.nfrac; ++f) {
+      z64 v[4];
+      v[1] = v[2] = v[3] = mkc(0.0, 0.0);
+      frac_direct(tb.frac[f], z, h, cplx, v);
+      frac_coefs<R>(tb, f, v, h, cplx, m, (size_t)M, coefs);
+    }
+  }
+  __syncthreads();
+  const int nsmall = *count;
+  if (nsmall > 0) {
+    constexpr int GS = T < 32 ? T : 32;
+    constexpr int NG = T / GS;
+    const int g = t / GS, lane = t % GS;
+    for (int base = 0; base < nsmall; base += NG) {
+      const int idx = base + g;
+      if (idx < nsmall) {
+        const int m = list[idx];
+        const z64 z = mode_z(L, L_complex, m, h);
+        for (int f = 0; f < tb.nfrac; ++f) {
+          const rks_frac fr = tb.frac[f];
+          z64 v[4];
+          v[1] = v[2] = v[3] = mkc(0.0, 0.0);
+          if (fr.mask & 14) {
+            z64 acc[4];
+            acc[1] = acc[2] = acc[3] = mkc(0.0, 0.0);
+            for (int j = lane; j < ct.contour_points; j += GS) {
+              const z64 nd = contour_node(ct, j);
+              frac_contour_add(fr, mkc(z.re + nd.re, z.im + nd.im), acc);
+            }
+#pragma unroll
+            for (int o = GS / 2; o >= 1; o >>= 1) {
+#pragma unroll
+              for (int r = 1; r <= 3; ++r) {
+                acc[r].re += __shfl_xor(acc[r].re, o, 64);
+                acc[r].im += __shfl_xor(acc[r].im, o, 64);
+              }
+            }
+            v[1] = np_rdiv(np_rmul(h, acc[1]), (double)ct.contour_points);
+            v[2] = np_rdiv(np_rmul(h, acc[2]), (double)ct.contour_points);
+            v[3] = np_rdiv(np_rmul(h, acc[3]), (double)ct.contour_points);
+          }
+          if (lane == 0) {
+            v[0] = np_exp(frac_arg(z, fr.fnum, fr.fden, cplx));
+            frac_coefs<R>(tb, f, v, h, cplx, m, (size_t)M, coefs);
+          }
+        }
+      }
+    }
+  }
+  __syncthreads();
 }
 
 // ---------------------------------------------------------------------------
@@ -210,6 +322,105 @@ __device__ __forceinline__ cx<R> stage_value(const rks_stage& S, const cx<R>* __
   return acc;
 }
 
+// Chunked, term-major forms of stage_value / error_value: CH modes
+// m = m0 + i*T of one thread at once, so every term issues CH independent
+// loads (the per-mode forms above wait on one mode at a time).
+template <typename R, int CH, typename SlotF>
+__device__ __forceinline__ void stage_chunk(const rks_stage& S, const cx<R>* __restrict__ u, SlotF slotp,
+                                            const cx<R>* __restrict__ coefs, size_t M, int m0, int T, int mlim,
+                                            double h, cx<R> (&acc)[CH]) {
+  using C = cx<R>;
+  const C* ec = S.ecoef < 0 ? nullptr : coefs + (size_t)S.ecoef * M;
+#pragma unroll
+  for (int i = 0; i < CH; ++i) {
+    const int m = m0 + i * T;
+    acc[i] = mkc((R)0, (R)0);
+    if (m < mlim) acc[i] = ec ? ec[m] * u[m] : u[m];
+  }
+  C sum[CH];
+#pragma unroll
+  for (int i = 0; i < CH; ++i) sum[i] = mkc((R)0, (R)0);
+  for (int j = 0; j < S.nterm; ++j) {
+    const rks_term& tm = S.t[j];
+    const C* nl = slotp(tm.src);
+    const C* nl2 = tm.src2 >= 0 ? slotp(tm.src2) : nullptr;
+    const C* cf = tm.coef >= 0 ? coefs + (size_t)tm.coef * M : nullptr;
+    const R s2 = (R)tm.scale2;
+    const R hr = (R)h;
+    const R sc = (R)tm.scale;
+    const R hs = (R)((tm.scale * h) / tm.den);
+#pragma unroll
+    for (int i = 0; i < CH; ++i) {
+      const int m = m0 + i * T;
+      if (m < mlim) {
+        C v = nl[m];
+        if (nl2) v = v + s2 * nl2[m];
+        C c;
+        if (tm.hmode == 0) {
+          c = cf ? cf[m] * v : v;
+          if (tm.scale != 1.0) c = sc * c;
+        } else if (tm.hmode == 1) {
+          c = cf ? (hr * cf[m]) * v : hr * v;
+          if (tm.scale != 1.0) c = sc * c;
+        } else {
+          c = hs * v;
+        }
+        if (S.hsum) sum[i] = (j == 0) ? c : sum[i] + c;
+        else acc[i] = acc[i] + c;
+      }
+    }
+  }
+  if (S.hsum) {
+#pragma unroll
+    for (int i = 0; i < CH; ++i) acc[i] = acc[i] + (R)h * sum[i];
+  }
+}
+
+template <typename R, int CH, typename SlotF>
+__device__ __forceinline__ void error_chunk(const rks_tableau& tb, SlotF slotp, const cx<R>* __restrict__ coefs,
+                                            size_t M, int m0, int T, int mlim, double h, cx<R> (&ev)[CH]) {
+  using C = cx<R>;
+#pragma unroll
+  for (int i = 0; i < CH; ++i) ev[i] = mkc((R)0, (R)0);
+  for (int j = 0; j < tb.nerr; ++j) {
+    const rks_term& tm = tb.err[j];
+    const C* nl = slotp(tm.src);
+    const C* nl2 = tm.src2 >= 0 ? slotp(tm.src2) : nullptr;
+    const C* cf = tm.coef >= 0 ? coefs + (size_t)tm.coef * M : nullptr;
+    const R s2 = (R)tm.scale2;
+    const R sc = (R)tm.scale;
+    const R hs = (R)((tm.scale * h) / tm.den);
+#pragma unroll
+    for (int i = 0; i < CH; ++i) {
+      const int m = m0 + i * T;
+      if (m < mlim) {
+        C v = nl[m];
+        if (nl2) v = v + s2 * nl2[m];
+        C c;
+        if (tm.hmode == 2) {
+          c = hs * v;
+        } else {
+          c = cf ? cf[m] * v : v;
+          if (tm.scale != 1.0) c = sc * c;
+        }
+        ev[i] = (j == 0) ? c : ev[i] + c;
+      }
+    }
+  }
+  if (tb.err_factored) {
+    const C* cf = tb.err_coef >= 0 ? coefs + (size_t)tb.err_coef * M : nullptr;
+#pragma unroll
+    for (int i = 0; i < CH; ++i) {
+      const int m = m0 + i * T;
+      if (m < mlim) {
+        if (tb.err_hpow) ev[i] = (R)h * ev[i];
+        if (tb.err_scale != 1.0) ev[i] = (R)tb.err_scale * ev[i];
+        if (cf) ev[i] = cf[m] * ev[i];
+      }
+    }
+  }
+}
+
 // embedded error estimate of the tableau (etd35.py:344, if34.py:130, if45dp.py:170-177)
 template <typename R, typename SlotF>
 __device__ __forceinline__ cx<R> error_value(const rks_tableau& tb, SlotF slotp, const cx<R>* __restrict__ coefs,
@@ -226,105 +437,141 @@ __device__ __forceinline__ cx<R> error_value(const rks_tableau& tb, SlotF slotp,
 }
 
 // ---------------------------------------------------------------------------
-// nonlinear term in LDS: fill from getk(m), transform, write dst (mode-owner
-// mapping m = tid + i*T is the same for fill and write-back)
+// nonlinear term in LDS.  fill_nl writes the spectrum produced by getk(m)
+// into the padded LDS buffer (inlined into the stage loop); nl_core runs the
+// transforms and writes NL to dst.  The mode-owner mapping m = tid + i*T is the
+// same for fill and write-back, so every global array element is produced and
+// consumed by one thread.  nl_core is not inlined: its register allocation is
+// independent of the controller and coefficient code around it.
 // ---------------------------------------------------------------------------
+// modes per chunk of the stage combination
+template <int MPT>
+struct Chunk {
+  static constexpr int CH = MPT < 8 ? MPT : 8;
+  static constexpr int NCH = (MPT + CH - 1) / CH;
+};
+
+// getk(m0, acc): the CH spectrum values of modes m0 + i*T (i < CH)
 template <typename R, int LOG2N, int T, int NLK, typename GetK>
-__device__ __forceinline__ void eval_nl(GetK getk, cx<R>* buf, const cx<R>* __restrict__ tw,
-                                        const R* __restrict__ kx, R coef, cx<R>* __restrict__ dst) {
+__device__ __forceinline__ void fill_nl(GetK getk, cx<R>* buf, const R* __restrict__ kx) {
   using C = cx<R>;
+  constexpr int N = 1 << LOG2N;
+  constexpr int M = (NLK == NLK_ADVECT) ? N / 2 + 1 : N;
+  constexpr int MPT = (M + T - 1) / T;
+  constexpr int CH = Chunk<MPT>::CH;
+  constexpr int H = N / 2;
+  const int t = threadIdx.x;
+#pragma unroll 1
+  for (int c = 0; c < Chunk<MPT>::NCH; ++c) {
+    const int m0 = t + c * CH * T;
+    C acc[CH];
+    getk(m0, acc);
+#pragma unroll
+    for (int i = 0; i < CH; ++i) {
+      const int m = m0 + i * T;
+      if (m >= M) continue;
+      if constexpr (NLK == NLK_CUBIC) {
+        buf[rksfft::pad(m)] = acc[i];
+      } else {
+        const C U = acc[i];
+        const R k = kx[m];
+        const C V = mkc(-k * U.im, k * U.re);
+        // spectrum of u + i u_x: Y[m] = U + iV, Y[N-m] = conj(U) + i conj(V); DC and
+        // Nyquist keep only the real parts, as irfft does
+        if (m == 0 || m == H) {
+          buf[rksfft::pad(m)] = mkc(U.re, V.re);
+        } else {
+          buf[rksfft::pad(m)] = mkc(U.re - V.im, U.im + V.re);
+          buf[rksfft::pad(N - m)] = mkc(U.re + V.im, V.re - U.im);
+        }
+      }
+    }
+  }
+}
+
+template <typename R, int LOG2N, int T, int NLK>
+__device__ __attribute__((noinline)) void nl_core(cx<R>* buf, const cx<R>* twl, R coef, cx<R>* __restrict__ dst) {
+  using C = cx<R>;
+  using rksfft::pad;
   constexpr int N = 1 << LOG2N;
   const int t = threadIdx.x;
   const R inv = (R)1 / (R)N;
+  __syncthreads();
   if constexpr (NLK == NLK_CUBIC) {
-#pragma unroll
-    for (int i = 0; i < N / T; ++i) {
-      const int m = t + i * T;
-      buf[m] = getk(m);
-    }
-    __syncthreads();
-    rksfft::fft<R, N, T, true>(buf, tw, 1);
+    rksfft::fft<R, N, T, true, LOG2N, RKS_MAXR>(buf, twl);
 #pragma unroll
     for (int i = 0; i < N / T; ++i) {
       const int j = t + i * T;
-      const C v = inv * buf[j];
+      const C v = inv * buf[pad(j)];
       const R a = v.re * v.re + v.im * v.im;
-      buf[j] = a * v;
+      buf[pad(j)] = a * v;
     }
     __syncthreads();
-    rksfft::fft<R, N, T, false>(buf, tw, 1);
+    rksfft::fft<R, N, T, false, LOG2N, RKS_MAXR>(buf, twl);
 #pragma unroll
     for (int i = 0; i < N / T; ++i) {
       const int m = t + i * T;
-      const C w = buf[m];
+      const C w = buf[pad(m)];
       dst[m] = mkc(-coef * w.im, coef * w.re);
     }
-    __syncthreads();
   } else {
     constexpr int M = N / 2 + 1;
     constexpr int MPT = (M + T - 1) / T;
     constexpr int H = N / 2;
-#pragma unroll
-    for (int i = 0; i < MPT; ++i) {
-      const int m = t + i * T;
-      if (m < M) {
-        const C U = getk(m);
-        const R k = kx[m];
-        const C V = mkc(-k * U.im, k * U.re);
-        if (m == 0 || m == H) {
-          buf[m] = mkc(U.re, V.re);
-        } else {
-          buf[m] = mkc(U.re - V.im, U.im + V.re);
-          buf[N - m] = mkc(U.re + V.im, V.re - U.im);
-        }
-      }
-    }
-    __syncthreads();
-    rksfft::fft<R, N, T, true>(buf, tw, 1);
+    rksfft::fft<R, N, T, true, LOG2N, RKS_MAXR>(buf, twl);
     C zz[H / T];
 #pragma unroll
     for (int i = 0; i < H / T; ++i) {
       const int j = t + i * T;
-      const C a = buf[2 * j], b = buf[2 * j + 1];
+      const C a = buf[pad(2 * j)], b = buf[pad(2 * j + 1)];
       zz[i] = mkc((a.re * inv) * (a.im * inv), (b.re * inv) * (b.im * inv));
     }
     __syncthreads();
 #pragma unroll
-    for (int i = 0; i < H / T; ++i) buf[t + i * T] = zz[i];
+    for (int i = 0; i < H / T; ++i) buf[pad(t + i * T)] = zz[i];
     __syncthreads();
-    rksfft::fft<R, H, T, false>(buf, tw, 2);
+    rksfft::fft<R, H, T, false, LOG2N, RKS_MAXR>(buf, twl);
 #pragma unroll
     for (int i = 0; i < MPT; ++i) {
       const int m = t + i * T;
       if (m < M) {
-        const C Zk = buf[m & (H - 1)];
-        const C Zc = conjc(buf[(H - m) & (H - 1)]);
+        const C Zk = buf[pad(m & (H - 1))];
+        const C Zc = conjc(buf[pad((H - m) & (H - 1))]);
         const C E = (R)0.5 * (Zk + Zc);
         const C D = Zk - Zc;
         const C O = mkc((R)0.5 * D.im, (R)-0.5 * D.re);
-        const C W = E + tw[m] * O;
+        const C W = E + rksfft::twiddle<R, LOG2N>(twl, m) * O;
         dst[m] = coef * W;
       }
     }
-    __syncthreads();
   }
+  __syncthreads();
+}
+
+template <typename R, int LOG2N, int T, int NLK, typename GetK>
+__device__ __forceinline__ void eval_nl(GetK getk, cx<R>* buf, const cx<R>* twl, const R* __restrict__ kx, R coef,
+                                        cx<R>* __restrict__ dst) {
+  fill_nl<R, LOG2N, T, NLK>(getk, buf, kx);
+  nl_core<R, LOG2N, T, NLK>(buf, twl, coef, dst);
 }
 
 // ---------------------------------------------------------------------------
 // one instance: evolve / step / nl / coefficients
 // ---------------------------------------------------------------------------
 template <typename R, int LOG2N, int T, int NLK>
-__device__ void run_instance(const rks_launch& P, int idx, cx<R>* buf, double* red) {
+__device__ void run_instance(const rks_launch& P, int idx, cx<R>* buf, const cx<R>* twl, double* red,
+                             int* lds_count) {
   using C = cx<R>;
   constexpr int N = 1 << LOG2N;
   constexpr int M = (NLK == NLK_ADVECT) ? N / 2 + 1 : N;
   constexpr int MPT = (M + T - 1) / T;
-  const rks_tableau& tb = *P.tab;
-  const rks_ctrl& ct = *P.ctl;
+  const rks_tableau* __restrict__ tbp = P.tab;
+  const rks_ctrl* __restrict__ ctp = P.ctl;
+  const rks_tableau& tb = *tbp;
+  const rks_ctrl& ct = *ctp;
   rks_inst* in = P.inst + idx;
   const rks_io io = P.io[idx];
   const int t = threadIdx.x;
-  const C* tw = (const C*)P.tw;
   const R* kx = (const R*)in->kx;
   const R nlc = (R)in->nl_coef;
   const double* Ld = in->L;
@@ -332,12 +579,20 @@ __device__ void run_instance(const rks_launch& P, int idx, cx<R>* buf, double* r
   C* ws = (C*)in->ws;
   const int nslot = tb.nslot;
   C* slot_base = ws;
-  C* candb[2] = {ws + (size_t)nslot * M, ws + (size_t)(nslot + 1) * M};
+  C* const cand0 = ws + (size_t)nslot * M;
+  C* const cand1 = ws + (size_t)(nslot + 1) * M;
   C* coefs = ws + (size_t)(nslot + 2) * M;
 
   if (P.mode == RKS_MODE_NL) {
     const C* u = (const C*)io.u_in;
-    eval_nl<R, LOG2N, T, NLK>([&](int m) { return u[m]; }, buf, tw, kx, nlc, (C*)io.u_out);
+    auto getu = [&](int m0, C(&acc)[Chunk<MPT>::CH]) {
+#pragma unroll
+      for (int i = 0; i < Chunk<MPT>::CH; ++i) {
+        const int m = m0 + i * T;
+        acc[i] = m < M ? u[m] : mkc((R)0, (R)0);
+      }
+    };
+    eval_nl<R, LOG2N, T, NLK>(getu, buf, twl, kx, nlc, (C*)io.u_out);
     return;
   }
   if (P.mode == RKS_MODE_COEFFS) {
@@ -355,17 +610,17 @@ __device__ void run_instance(const rks_launch& P, int idx, cx<R>* buf, double* r
   int rejects = in->rejects, status = in->status, ubuf = in->ubuf;
   int log_count = in->log_count, snap_count = in->snap_count;
   int nl_calls = in->nl_calls, coef_updates = in->coef_updates;
-  int smap[RKS_MAX_SLOTS];
+  // slot map packed 4 bits per slot (no dynamically indexed register array)
+  unsigned smap = 0;
 #pragma unroll
-  for (int s = 0; s < RKS_MAX_SLOTS; ++s) smap[s] = in->slot_map[s];
+  for (int s = 0; s < RKS_MAX_SLOTS; ++s) smap |= (unsigned)(in->slot_map[s] & 15) << (4 * s);
   const int store_freq = in->store_freq, snap_cap = in->snap_cap, log_cap = in->log_cap;
 
   if (io.reset) {
     started = 0;
     accept = 0;
     h_coeff = __builtin_nan("");
-#pragma unroll
-    for (int s = 0; s < RKS_MAX_SLOTS; ++s) smap[s] = s;
+    smap = 0x76543210u;
   }
   const bool evolve = (P.mode == RKS_MODE_EVOLVE);
   const C* uin = (const C*)io.u_in;
@@ -395,8 +650,8 @@ __device__ void run_instance(const rks_launch& P, int idx, cx<R>* buf, double* r
     attempts = rejects = 0;
   }
 
-  auto slot = [&](int s) -> C* { return slot_base + (size_t)smap[s] * M; };
-  auto cur_u = [&]() -> const C* { return ubuf < 0 ? uin : (const C*)candb[ubuf]; };
+  auto slot = [&](int s) -> C* { return slot_base + (size_t)((smap >> (4 * s)) & 15u) * M; };
+  auto cur_u = [&]() -> const C* { return ubuf < 0 ? uin : (const C*)(ubuf == 0 ? cand0 : cand1); };
 
   bool done = false;
   while (!done) {
@@ -411,48 +666,56 @@ __device__ void run_instance(const rks_launch& P, int idx, cx<R>* buf, double* r
       }
     }
     const C* u = evolve ? cur_u() : uin;
-    C* cand = evolve ? candb[ubuf == 0 ? 1 : 0] : (C*)io.u_out;
+    C* cand = evolve ? (ubuf == 0 ? cand1 : cand0) : (C*)io.u_out;
     int numloops = 0;
     double h_suggest = h;
     // ---------------- attempt loop (solveras.py:384-410) ----------------
     for (;;) {
       if (h != h_coeff) {
-        for (int i = 0; i < MPT; ++i) {
-          const int m = t + i * T;
-          if (m < M) coef_mode<R>(tb, ct, Ld, Lc, m, M, h, coefs);
-        }
+        coef_update<R, T, MPT>(tb, ct, Ld, Lc, M, h, coefs, (int*)buf, lds_count);
         h_coeff = h;
         ++coef_updates;
       }
+      auto getu = [&](int m0, C(&acc)[Chunk<MPT>::CH]) {
+#pragma unroll
+        for (int i = 0; i < Chunk<MPT>::CH; ++i) {
+          const int m = m0 + i * T;
+          acc[i] = m < M ? u[m] : mkc((R)0, (R)0);
+        }
+      };
       if (!started) {
-        eval_nl<R, LOG2N, T, NLK>([&](int m) { return u[m]; }, buf, tw, kx, nlc, slot(0));
+        eval_nl<R, LOG2N, T, NLK>(getu, buf, twl, kx, nlc, slot(0));
         ++nl_calls;
         started = 1;
       } else if (accept || !tb.adaptive) {
         if (tb.nl1_policy == RKS_NL1_RECOMPUTE) {
-          eval_nl<R, LOG2N, T, NLK>([&](int m) { return u[m]; }, buf, tw, kx, nlc, slot(0));
+          eval_nl<R, LOG2N, T, NLK>(getu, buf, twl, kx, nlc, slot(0));
           ++nl_calls;
         } else {
-          const int a = smap[0];
-          smap[0] = smap[tb.fsal_slot];
-          smap[tb.fsal_slot] = a;
+          const unsigned f4 = 4u * (unsigned)tb.fsal_slot;
+          const unsigned a = smap & 15u, b = (smap >> f4) & 15u;
+          smap = (smap & ~(15u | (15u << f4))) | b | (a << f4);
         }
       }
       for (int s = 0; s < tb.nstage; ++s) {
         const rks_stage& S = tb.st[s];
         const bool last = (s == tb.nstage - 1);
-        auto getk = [&](int m) -> C {
-          const C acc = stage_value<R>(S, u, slot, coefs, M, m, h);
-          if (last) cand[m] = acc;
-          return acc;
+        auto getk = [&](int m0, C(&acc)[Chunk<MPT>::CH]) {
+          stage_chunk<R, Chunk<MPT>::CH>(S, u, slot, coefs, (size_t)M, m0, T, M, h, acc);
+          if (last) {
+#pragma unroll
+            for (int i = 0; i < Chunk<MPT>::CH; ++i)
+              if (m0 + i * T < M) cand[m0 + i * T] = acc[i];
+          }
         };
         if (S.has_nl) {
-          eval_nl<R, LOG2N, T, NLK>(getk, buf, tw, kx, nlc, slot(S.nl_slot));
+          eval_nl<R, LOG2N, T, NLK>(getk, buf, twl, kx, nlc, slot(S.nl_slot));
           ++nl_calls;
         } else {
-          for (int i = 0; i < MPT; ++i) {
-            const int m = t + i * T;
-            if (m < M) (void)getk(m);
+#pragma unroll 1
+          for (int c = 0; c < Chunk<MPT>::NCH; ++c) {
+            C acc[Chunk<MPT>::CH];
+            getk(t + c * Chunk<MPT>::CH * T, acc);
           }
         }
       }
@@ -462,34 +725,36 @@ __device__ void run_instance(const rks_launch& P, int idx, cx<R>* buf, double* r
         break;
       }
       // ---------------- embedded error norm (solveras.py:449-455) ----------------
-      double kmag[MPT], k2[MPT], e2[MPT];
+      // pass 1: max |u_new|; pass 2: masked sums of |u_new|^2 and |err|^2 (FP64)
       double mx = 0.0;
-#pragma unroll
       for (int i = 0; i < MPT; ++i) {
         const int m = t + i * T;
-        kmag[i] = 0.0;
-        k2[i] = 0.0;
-        e2[i] = 0.0;
         if (m < M) {
           const C kv = cand[m];
-          const C ev = error_value<R>(tb, slot, coefs, M, m, h);
-          const double kr = (double)kv.re, ki = (double)kv.im;
-          const double er = (double)ev.re, ei = (double)ev.im;
-          kmag[i] = hypot(kr, ki);
-          k2[i] = kr * kr + ki * ki;
-          e2[i] = er * er + ei * ei;
-          mx = (i == 0) ? kmag[i] : nanmax(mx, kmag[i]);
+          const double a = hypot((double)kv.re, (double)kv.im);
+          mx = (i == 0) ? a : nanmax(mx, a);
         }
       }
-      if (t >= M) mx = 0.0;
       mx = wg_max<T>(mx, red);
       double su = 0.0, se = 0.0;
+#pragma unroll 1
+      for (int c = 0; c < Chunk<MPT>::NCH; ++c) {
+        constexpr int CH = Chunk<MPT>::CH;
+        const int m0 = t + c * CH * T;
+        C ev[CH];
+        error_chunk<R, CH>(tb, slot, coefs, (size_t)M, m0, T, M, h, ev);
 #pragma unroll
-      for (int i = 0; i < MPT; ++i) {
-        const int m = t + i * T;
-        if (m < M && (kmag[i] / mx > ct.adapt_cutoff)) {
-          su += k2[i];
-          se += e2[i];
+        for (int i = 0; i < CH; ++i) {
+          const int m = m0 + i * T;
+          if (m < M) {
+            const C kv = cand[m];
+            const double kr = (double)kv.re, ki = (double)kv.im;
+            if (hypot(kr, ki) / mx > ct.adapt_cutoff) {
+              const double er = (double)ev[i].re, ei = (double)ev[i].im;
+              su += kr * kr + ki * ki;
+              se += er * er + ei * ei;
+            }
+          }
         }
       }
       wg_sum2<T>(su, se, red);
@@ -590,24 +855,32 @@ __device__ void run_instance(const rks_launch& P, int idx, cx<R>* buf, double* r
     in->nl_calls = nl_calls;
     in->coef_updates = coef_updates;
 #pragma unroll
-    for (int s = 0; s < RKS_MAX_SLOTS; ++s) in->slot_map[s] = smap[s];
+    for (int s = 0; s < RKS_MAX_SLOTS; ++s) in->slot_map[s] = (int)((smap >> (4 * s)) & 15u);
   }
 }
 
+// waves per SIMD the register allocation must allow: the FFT core fits in
+// 256 (n = 8192, 512 threads) or 168 registers; the rarely executed FP64
+// coefficient code spills instead of lowering occupancy.
+__host__ __device__ constexpr int rks_min_waves(int log2n) { return log2n >= 13 ? 2 : 3; }
+
 template <typename R, int LOG2N, int NLK>
-__global__ __launch_bounds__(rks_threads(LOG2N, NLK)) void rks_kernel(rks_launch P) {
+__global__ __launch_bounds__(rks_threads(LOG2N, NLK), rks_min_waves(LOG2N)) void rks_kernel(rks_launch P) {
   constexpr int T = rks_threads(LOG2N, NLK);
   extern __shared__ __align__(16) unsigned char rks_smem[];
   __shared__ double red[2 * (T / 64 + 1)];
   __shared__ int s_item;
+  __shared__ int s_count;
   cx<R>* buf = (cx<R>*)rks_smem;
+  cx<R>* twl = buf + rksfft::padded_size(1 << LOG2N);
+  rksfft::load_twiddles<R, LOG2N, T>(twl, (const cx<R>*)P.tw);
   for (;;) {
     if (threadIdx.x == 0) s_item = atomicAdd(P.queue, 1);
     __syncthreads();
     const int item = s_item;
     __syncthreads();
     if (item >= P.count) break;
-    run_instance<R, LOG2N, T, NLK>(P, P.order[item], buf, red);
+    run_instance<R, LOG2N, T, NLK>(P, P.order[item], buf, twl, red, &s_count);
     __syncthreads();
   }
 }

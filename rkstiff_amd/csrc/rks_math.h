@@ -111,9 +111,8 @@ __device__ __forceinline__ z64 np_exp(z64 z) {
 //   psi1 = (exp(w) - 1) / w
 //   psi2 = 2 * (exp(w) - 1 - w) / w**2
 //   psi3 = 6 * (exp(w) - 1 - w - w**2 / 2) / w**3
-__device__ __forceinline__ void np_psi123(z64 w, int need3, z64* p1, z64* p2, z64* p3) {
+__device__ __forceinline__ void np_psi123_e(z64 w, z64 e, int need3, z64* p1, z64* p2, z64* p3) {
 #pragma clang fp contract(off)
-  z64 e = np_exp(w);
   z64 em1 = mkc(e.re - 1.0, e.im);
   *p1 = np_div(em1, w);
   z64 w2 = np_mul(w, w);
@@ -125,4 +124,7 @@ __device__ __forceinline__ void np_psi123(z64 w, int need3, z64* p1, z64* p2, z6
     z64 t3 = mkc(t2.re - h2.re, t2.im - h2.im);
     *p3 = np_div(np_rmul(6.0, t3), w3);
   }
+}
+__device__ __forceinline__ void np_psi123(z64 w, int need3, z64* p1, z64* p2, z64* p3) {
+  np_psi123_e(w, np_exp(w), need3, p1, p2, p3);
 }

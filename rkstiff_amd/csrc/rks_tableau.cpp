@@ -29,23 +29,29 @@ struct Builder {
     t.fsal_slot = fsal_slot;
     t.uses_contour = contour;
   }
+  // basis handle = fraction index * 4 + kind
   int basis(int r, int fnum, int fden) {
-    for (int i = 0; i < t.nbasis; ++i)
-      if (t.basis[i].r == r && t.basis[i].fnum == fnum && t.basis[i].fden == fden) return i;
-    rks_basis& b = t.basis[t.nbasis];
-    b.r = r;
-    b.fnum = fnum;
-    b.fden = fden;
-    return t.nbasis++;
+    int f = 0;
+    for (; f < t.nfrac; ++f)
+      if (t.frac[f].fnum == fnum && t.frac[f].fden == fden) break;
+    if (f == t.nfrac) {
+      t.frac[f].fnum = fnum;
+      t.frac[f].fden = fden;
+      t.frac[f].mask = 0;
+      ++t.nfrac;
+    }
+    t.frac[f].mask |= 1 << r;
+    return 4 * f + r;
   }
-  // c = (sum w_k B_k) / den  (B_k already carries h for psi bases)
+  // c = (outer * sum w_k B_k) / den  (psi bases already carry h); one fraction per recipe
   int coef(int nb, const int* bs, const double* ws, double den, int hmul = 0, double outer = 1.0) {
     rks_coef& c = t.coef[t.ncoef];
     c.nb = nb;
-    c.outer = outer;
     c.hmul = hmul;
+    c.outer = outer;
+    c.frac = bs[0] / 4;
     for (int i = 0; i < nb; ++i) {
-      c.basis[i] = bs[i];
+      c.kind[i] = bs[i] % 4;
       c.w[i] = ws[i];
     }
     c.den = den;
@@ -97,6 +103,7 @@ struct Builder {
     t.err_hpow = hpow;
   }
   void err(const rks_term& x) { t.err[t.nerr++] = x; }
+  rks_tableau finalize() { return t; }
 };
 
 // ETD35 / ETD5 share the fifth-order coefficient rows (etd35.py:213-237).
@@ -216,7 +223,7 @@ rks_tableau make_etd35() {
   B.err(Builder::mk(3, RKS_COEF_ONE, -6.0, 0, -1, 0.0, 1.0));
   B.err(Builder::mk(4, RKS_COEF_ONE, 4.0, 0, -1, 0.0, 1.0));
   B.err(Builder::mk(5, RKS_COEF_ONE, -1.0, 0, -1, 0.0, 1.0));
-  return B.t;
+  return B.finalize();
 }
 
 rks_tableau make_etd5() {
@@ -229,7 +236,7 @@ rks_tableau make_etd5() {
   Builder::term(s7, 3, F.a74);            // b4
   Builder::term(s7, 4, F.a75);            // b5
   Builder::term(s7, 5, F.a76);            // b6
-  return B.t;
+  return B.finalize();
 }
 
 struct Fourth {
@@ -296,14 +303,14 @@ rks_tableau make_etd34() {
   fourth_stages(B, F, 1);
   B.factored_err(F.a54, 1.0, 0);  // err = a54 (NL4 - NL5)  (etd34.py:190)
   B.err(Builder::mk(3, RKS_COEF_ONE, 1.0, 0, 4, -1.0, 1.0));
-  return B.t;
+  return B.finalize();
 }
 
 rks_tableau make_etd4() {
   Builder B(2, 0, 4, RKS_NL1_FSAL, 5, 4, 1);
   Fourth F = fourth_coeffs(B);
   fourth_stages(B, F, 1);
-  return B.t;
+  return B.finalize();
 }
 
 // IF34 / IF4: integrating-factor RK4 (if34.py:120-130, if4.py:112-121)
@@ -327,7 +334,7 @@ rks_tableau make_if4family(int method, int adaptive) {
     B.factored_err(RKS_COEF_ONE, 1.0 / 6.0, 1);  // err = h (NL4 - NL5) / 6.0
     B.err(Builder::mk(3, RKS_COEF_ONE, 1.0, 0, 4, -1.0, 1.0));
   }
-  return B.t;
+  return B.finalize();
 }
 
 // IF45DP: Dormand-Prince in the integrating-factor frame (if45dp.py:140-238)
@@ -392,7 +399,7 @@ rks_tableau make_if45dp() {
   B.err(Builder::mk(4, r5, 1.0, 0, -1, 0.0, 1.0));
   B.err(Builder::mk(5, RKS_COEF_ONE, 22.0, 2, -1, 0.0, 525.0));  // r6 = 22 * h / 525.0
   B.err(Builder::mk(6, RKS_COEF_ONE, -1.0, 2, -1, 0.0, 40.0));   // r7 = -h / 40.0
-  return B.t;
+  return B.finalize();
 }
 
 }  // namespace

@@ -12,11 +12,11 @@
 #define RKS_MAX_STAGES 8
 #define RKS_MAX_TERMS 8
 #define RKS_MAX_COEF 32
-#define RKS_MAX_BASIS 16
+#define RKS_MAX_FRAC 16
 #define RKS_MAX_SLOTS 8
 #define RKS_COEF_ONE (-1)
 
-// basis value kinds (rks_basis.r)
+// basis value kinds (rks_coef.kind)
 #define RKS_BASIS_EXP 0  // exp(f z)
 #define RKS_BASIS_PSI1 1 // h * psi1(f z)   (psi_r = r! phi_r)
 #define RKS_BASIS_PSI2 2
@@ -64,19 +64,24 @@ typedef struct rks_stage {
   rks_term t[RKS_MAX_TERMS];
 } rks_stage;
 
-typedef struct rks_basis {
-  int r;       // RKS_BASIS_*
-  int fnum;    // argument f z with f = fnum / fden
+// One argument fraction f = fnum/fden of z = h L.  mask: bit r set when basis
+// kind r (RKS_BASIS_*) of this fraction is used by some coefficient.
+typedef struct rks_frac {
+  int fnum;
   int fden;
+  int mask;
   int pad;
-} rks_basis;
+} rks_frac;
 
+// c = (outer * sum_k w_k B(frac, kind_k)) / den; every recipe uses one fraction
 typedef struct rks_coef {
   int nb;
   int hmul;            // weights multiplied by h (IF45DP a_ij = w h exp(f z))
-  int basis[4];
+  int frac;
+  int pad;
+  int kind[4];
   double w[4];
-  double outer;        // c = (outer * (sum_k w_k B_k)) / den
+  double outer;
   double den;
 } rks_coef;
 
@@ -90,7 +95,7 @@ typedef struct rks_tableau {
   int nstage;
   int nerr;
   int ncoef;
-  int nbasis;
+  int nfrac;
   int uses_contour;    // ETD methods: small modes through the contour mean
   int err_coef;        // factored error estimate: err = C[err_coef] * (((h^err_hpow) * sum(terms)) * err_scale)
   int err_factored;    //   (0: err = sum of the terms)
@@ -100,7 +105,7 @@ typedef struct rks_tableau {
   rks_stage st[RKS_MAX_STAGES];
   rks_term err[RKS_MAX_TERMS];
   rks_coef coef[RKS_MAX_COEF];
-  rks_basis basis[RKS_MAX_BASIS];
+  rks_frac frac[RKS_MAX_FRAC];
 } rks_tableau;
 
 typedef struct rks_ctrl {

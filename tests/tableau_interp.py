@@ -11,40 +11,41 @@ from oracle import solvers
 from rkstiff_amd import _lib
 
 
-def basis_values(t, L, h, etd):
+def frac_values(t, L, h, etd):
+    """Per fraction: [exp(f z), h psi1(f z), h psi2(f z), h psi3(f z)] over all modes."""
     z = h * L.astype(np.complex128)
     small = np.zeros(z.shape, bool)
     if t.uses_contour:
         small = np.abs(z) < etd.modecutoff
     nodes = solvers.contour_nodes(etd)
     out = []
-    for b in range(t.nbasis):
-        bs = t.basis[b]
-        f = bs.fnum / bs.fden
-        if bs.r == 0:
-            out.append(np.exp(bs.fnum * z / bs.fden))
-            continue
-        psi = (solvers.psi1, solvers.psi2, solvers.psi3)[bs.r - 1]
-        v = np.zeros(z.shape, np.complex128)
-        big = ~small
-        with np.errstate(all="ignore"):
-            v[big] = h * psi(f * z[big])
-            if np.any(small):
-                Z = z[small][:, None] + nodes[None, :]
-                v[small] = h * np.mean(psi(f * Z), axis=1)
-        out.append(v)
+    for f in range(t.nfrac):
+        fr = t.frac[f]
+        fv = fr.fnum / fr.fden
+        vals = [np.exp(fr.fnum * z / fr.fden)]
+        for psi in (solvers.psi1, solvers.psi2, solvers.psi3):
+            v = np.zeros(z.shape, np.complex128)
+            big = ~small
+            with np.errstate(all="ignore"):
+                v[big] = h * psi(fv * z[big])
+                if np.any(small):
+                    Z = z[small][:, None] + nodes[None, :]
+                    v[small] = h * np.mean(psi(fv * Z), axis=1)
+            vals.append(v)
+        out.append(vals)
     return out
 
 
 def coefficients(t, L, h, etd):
-    B = basis_values(t, L, h, etd)
+    F = frac_values(t, L, h, etd)
     cs = []
     for c in range(t.ncoef):
         cf = t.coef[c]
         acc = 0
         for k in range(cf.nb):
             w = cf.w[k] * (h if cf.hmul else 1.0)
-            bv = B[cf.basis[k]] if w == 1.0 else w * B[cf.basis[k]]
+            bv = F[cf.frac][cf.kind[k]]
+            bv = bv if w == 1.0 else w * bv
             acc = bv if k == 0 else acc + bv
         if cf.outer != 1.0:
             acc = cf.outer * acc
