@@ -51,7 +51,12 @@ enum {
                              and the error norm run here; n = number of modes */
 };
 
-enum { RKS_F64 = 0, RKS_F32 = 1 };
+/* engine precision: state/FFT/propagators + NL and coefficient storage
+ *   RKS_F64    all FP64
+ *   RKS_F32    all FP32 (coefficients generated in FP64; norm and dt control FP64)
+ *   RKS_MIXED  FP64 state, propagators exp(f h L) and FFTs; FP32 NL arrays and other
+ *              coefficients.  State pointers are complex128. */
+enum { RKS_F64 = 0, RKS_F32 = 1, RKS_MIXED = 2 };
 
 /* SolverConfig (solveras.py:27-169), the class constants MAX_LOOPS/MAX_S/MIN_S
  * (solveras.py:255-257) and ETDConfig (etd.py:46-131), passed by value. */

@@ -14,7 +14,7 @@ LIB_PATH = os.path.join(HERE, "librkstiff_amd.so")
 RKS_OK, RKS_ERR_INVALID, RKS_ERR_HIP, RKS_MAXLOOPS, RKS_MINSTEP, RKS_PAUSED, RKS_RUNNING = range(7)
 METHOD_IDS = {"ETD35": 0, "ETD34": 1, "ETD4": 2, "ETD5": 3, "IF34": 4, "IF45DP": 5, "IF4": 6}
 NL_ADVECT_RFFT, NL_CUBIC_FFT, NL_NONE = 0, 1, 2
-F64, F32 = 0, 1
+F64, F32, MIXED = 0, 1, 2
 
 EXPORTS = (
     "rks_plan_create", "rks_plan_destroy", "rks_plan_info", "rks_set_storage", "rks_evolve",
@@ -78,7 +78,9 @@ class Tableau(ctypes.Structure):
     _fields_ = [("method", ctypes.c_int), ("adaptive", ctypes.c_int), ("q", ctypes.c_int),
                 ("nl1_policy", ctypes.c_int), ("fsal_slot", ctypes.c_int), ("nslot", ctypes.c_int),
                 ("nstage", ctypes.c_int), ("nerr", ctypes.c_int), ("ncoef", ctypes.c_int),
-                ("nfrac", ctypes.c_int), ("uses_contour", ctypes.c_int), ("err_coef", ctypes.c_int),
+                ("nfrac", ctypes.c_int), ("uses_contour", ctypes.c_int),
+                ("coef_rs", ctypes.c_int * MAX_COEF), ("coef_off", ctypes.c_int * MAX_COEF),
+                ("err_coef", ctypes.c_int),
                 ("err_factored", ctypes.c_int), ("err_hpow", ctypes.c_int), ("pad2", ctypes.c_int),
                 ("err_scale", ctypes.c_double),
                 ("st", Stage * MAX_STAGES), ("err", Term * MAX_TERMS),

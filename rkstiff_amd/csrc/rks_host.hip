@@ -16,16 +16,16 @@
 // ---------------------------------------------------------------------------
 template <typename R>
 __global__ void rks_coef_kernel(const rks_tableau* tab, const rks_ctrl* ctl, const double* L, int Lc, int M,
-                                double h, cx<R>* coefs) {
+                                double h, char* cbase) {
   for (int m = blockIdx.x * blockDim.x + threadIdx.x; m < M; m += gridDim.x * blockDim.x)
-    coef_mode<R>(*tab, *ctl, L, Lc, m, M, h, coefs);
+    coef_mode<R, R>(*tab, *ctl, L, Lc, m, M, h, cbase);
 }
 
 struct rks_combine_args {
   const rks_tableau* tab;
   const void* u;
   const void* slots[RKS_MAX_SLOTS];
-  const void* coefs;
+  const char* coefs;
   void* out;
   int stage;
   int is_err;
@@ -38,12 +38,13 @@ template <typename R>
 __global__ void rks_combine_kernel(rks_combine_args A) {
   using C = cx<R>;
   const C* u = (const C*)A.u;
-  const C* cf = (const C*)A.coefs;
   C* out = (C*)A.out;
   auto slotp = [&](int s) -> const C* { return (const C*)A.slots[s]; };
   for (int m = blockIdx.x * blockDim.x + threadIdx.x; m < A.M; m += gridDim.x * blockDim.x) {
-    if (A.is_err) out[m] = error_value<R>(*A.tab, slotp, cf, (size_t)A.M, m, A.h);
-    else out[m] = stage_value<R>(A.tab->st[A.stage], u, slotp, cf, (size_t)A.M, m, A.h);
+    C v[1];
+    if (A.is_err) error_chunk<R, R, 1>(*A.tab, slotp, A.coefs, (size_t)A.M, m, 1, A.M, A.h, v);
+    else stage_chunk<R, R, 1>(*A.tab, A.tab->st[A.stage], u, slotp, A.coefs, (size_t)A.M, m, 1, A.M, A.h, v);
+    out[m] = v[0];
   }
 }
 
@@ -91,46 +92,67 @@ __global__ void rks_f64_to_f32(const double* a, float* b, int n) {
 
 
 // explicit instantiations are in rks_k_<prec>_<nl>.hip
-extern template __global__ void rks_kernel<double, 4, NLK_ADVECT>(rks_launch);
-extern template __global__ void rks_kernel<double, 5, NLK_ADVECT>(rks_launch);
-extern template __global__ void rks_kernel<double, 6, NLK_ADVECT>(rks_launch);
-extern template __global__ void rks_kernel<double, 7, NLK_ADVECT>(rks_launch);
-extern template __global__ void rks_kernel<double, 8, NLK_ADVECT>(rks_launch);
-extern template __global__ void rks_kernel<double, 9, NLK_ADVECT>(rks_launch);
-extern template __global__ void rks_kernel<double, 10, NLK_ADVECT>(rks_launch);
-extern template __global__ void rks_kernel<double, 11, NLK_ADVECT>(rks_launch);
-extern template __global__ void rks_kernel<double, 12, NLK_ADVECT>(rks_launch);
-extern template __global__ void rks_kernel<double, 13, NLK_ADVECT>(rks_launch);
-extern template __global__ void rks_kernel<double, 4, NLK_CUBIC>(rks_launch);
-extern template __global__ void rks_kernel<double, 5, NLK_CUBIC>(rks_launch);
-extern template __global__ void rks_kernel<double, 6, NLK_CUBIC>(rks_launch);
-extern template __global__ void rks_kernel<double, 7, NLK_CUBIC>(rks_launch);
-extern template __global__ void rks_kernel<double, 8, NLK_CUBIC>(rks_launch);
-extern template __global__ void rks_kernel<double, 9, NLK_CUBIC>(rks_launch);
-extern template __global__ void rks_kernel<double, 10, NLK_CUBIC>(rks_launch);
-extern template __global__ void rks_kernel<double, 11, NLK_CUBIC>(rks_launch);
-extern template __global__ void rks_kernel<double, 12, NLK_CUBIC>(rks_launch);
-extern template __global__ void rks_kernel<double, 13, NLK_CUBIC>(rks_launch);
-extern template __global__ void rks_kernel<float, 4, NLK_ADVECT>(rks_launch);
-extern template __global__ void rks_kernel<float, 5, NLK_ADVECT>(rks_launch);
-extern template __global__ void rks_kernel<float, 6, NLK_ADVECT>(rks_launch);
-extern template __global__ void rks_kernel<float, 7, NLK_ADVECT>(rks_launch);
-extern template __global__ void rks_kernel<float, 8, NLK_ADVECT>(rks_launch);
-extern template __global__ void rks_kernel<float, 9, NLK_ADVECT>(rks_launch);
-extern template __global__ void rks_kernel<float, 10, NLK_ADVECT>(rks_launch);
-extern template __global__ void rks_kernel<float, 11, NLK_ADVECT>(rks_launch);
-extern template __global__ void rks_kernel<float, 12, NLK_ADVECT>(rks_launch);
-extern template __global__ void rks_kernel<float, 13, NLK_ADVECT>(rks_launch);
-extern template __global__ void rks_kernel<float, 4, NLK_CUBIC>(rks_launch);
-extern template __global__ void rks_kernel<float, 5, NLK_CUBIC>(rks_launch);
-extern template __global__ void rks_kernel<float, 6, NLK_CUBIC>(rks_launch);
-extern template __global__ void rks_kernel<float, 7, NLK_CUBIC>(rks_launch);
-extern template __global__ void rks_kernel<float, 8, NLK_CUBIC>(rks_launch);
-extern template __global__ void rks_kernel<float, 9, NLK_CUBIC>(rks_launch);
-extern template __global__ void rks_kernel<float, 10, NLK_CUBIC>(rks_launch);
-extern template __global__ void rks_kernel<float, 11, NLK_CUBIC>(rks_launch);
-extern template __global__ void rks_kernel<float, 12, NLK_CUBIC>(rks_launch);
-extern template __global__ void rks_kernel<float, 13, NLK_CUBIC>(rks_launch);
+extern template __global__ void rks_kernel<double, double, 4, NLK_ADVECT>(rks_launch);
+extern template __global__ void rks_kernel<double, double, 5, NLK_ADVECT>(rks_launch);
+extern template __global__ void rks_kernel<double, double, 6, NLK_ADVECT>(rks_launch);
+extern template __global__ void rks_kernel<double, double, 7, NLK_ADVECT>(rks_launch);
+extern template __global__ void rks_kernel<double, double, 8, NLK_ADVECT>(rks_launch);
+extern template __global__ void rks_kernel<double, double, 9, NLK_ADVECT>(rks_launch);
+extern template __global__ void rks_kernel<double, double, 10, NLK_ADVECT>(rks_launch);
+extern template __global__ void rks_kernel<double, double, 11, NLK_ADVECT>(rks_launch);
+extern template __global__ void rks_kernel<double, double, 12, NLK_ADVECT>(rks_launch);
+extern template __global__ void rks_kernel<double, double, 13, NLK_ADVECT>(rks_launch);
+extern template __global__ void rks_kernel<double, double, 4, NLK_CUBIC>(rks_launch);
+extern template __global__ void rks_kernel<double, double, 5, NLK_CUBIC>(rks_launch);
+extern template __global__ void rks_kernel<double, double, 6, NLK_CUBIC>(rks_launch);
+extern template __global__ void rks_kernel<double, double, 7, NLK_CUBIC>(rks_launch);
+extern template __global__ void rks_kernel<double, double, 8, NLK_CUBIC>(rks_launch);
+extern template __global__ void rks_kernel<double, double, 9, NLK_CUBIC>(rks_launch);
+extern template __global__ void rks_kernel<double, double, 10, NLK_CUBIC>(rks_launch);
+extern template __global__ void rks_kernel<double, double, 11, NLK_CUBIC>(rks_launch);
+extern template __global__ void rks_kernel<double, double, 12, NLK_CUBIC>(rks_launch);
+extern template __global__ void rks_kernel<double, double, 13, NLK_CUBIC>(rks_launch);
+extern template __global__ void rks_kernel<float, float, 4, NLK_ADVECT>(rks_launch);
+extern template __global__ void rks_kernel<float, float, 5, NLK_ADVECT>(rks_launch);
+extern template __global__ void rks_kernel<float, float, 6, NLK_ADVECT>(rks_launch);
+extern template __global__ void rks_kernel<float, float, 7, NLK_ADVECT>(rks_launch);
+extern template __global__ void rks_kernel<float, float, 8, NLK_ADVECT>(rks_launch);
+extern template __global__ void rks_kernel<float, float, 9, NLK_ADVECT>(rks_launch);
+extern template __global__ void rks_kernel<float, float, 10, NLK_ADVECT>(rks_launch);
+extern template __global__ void rks_kernel<float, float, 11, NLK_ADVECT>(rks_launch);
+extern template __global__ void rks_kernel<float, float, 12, NLK_ADVECT>(rks_launch);
+extern template __global__ void rks_kernel<float, float, 13, NLK_ADVECT>(rks_launch);
+extern template __global__ void rks_kernel<float, float, 4, NLK_CUBIC>(rks_launch);
+extern template __global__ void rks_kernel<float, float, 5, NLK_CUBIC>(rks_launch);
+extern template __global__ void rks_kernel<float, float, 6, NLK_CUBIC>(rks_launch);
+extern template __global__ void rks_kernel<float, float, 7, NLK_CUBIC>(rks_launch);
+extern template __global__ void rks_kernel<float, float, 8, NLK_CUBIC>(rks_launch);
+extern template __global__ void rks_kernel<float, float, 9, NLK_CUBIC>(rks_launch);
+extern template __global__ void rks_kernel<float, float, 10, NLK_CUBIC>(rks_launch);
+extern template __global__ void rks_kernel<float, float, 11, NLK_CUBIC>(rks_launch);
+extern template __global__ void rks_kernel<float, float, 12, NLK_CUBIC>(rks_launch);
+extern template __global__ void rks_kernel<float, float, 13, NLK_CUBIC>(rks_launch);
+extern template __global__ void rks_kernel<double, float, 4, NLK_ADVECT>(rks_launch);
+extern template __global__ void rks_kernel<double, float, 5, NLK_ADVECT>(rks_launch);
+extern template __global__ void rks_kernel<double, float, 6, NLK_ADVECT>(rks_launch);
+extern template __global__ void rks_kernel<double, float, 7, NLK_ADVECT>(rks_launch);
+extern template __global__ void rks_kernel<double, float, 8, NLK_ADVECT>(rks_launch);
+extern template __global__ void rks_kernel<double, float, 9, NLK_ADVECT>(rks_launch);
+extern template __global__ void rks_kernel<double, float, 10, NLK_ADVECT>(rks_launch);
+extern template __global__ void rks_kernel<double, float, 11, NLK_ADVECT>(rks_launch);
+extern template __global__ void rks_kernel<double, float, 12, NLK_ADVECT>(rks_launch);
+extern template __global__ void rks_kernel<double, float, 13, NLK_ADVECT>(rks_launch);
+extern template __global__ void rks_kernel<double, float, 4, NLK_CUBIC>(rks_launch);
+extern template __global__ void rks_kernel<double, float, 5, NLK_CUBIC>(rks_launch);
+extern template __global__ void rks_kernel<double, float, 6, NLK_CUBIC>(rks_launch);
+extern template __global__ void rks_kernel<double, float, 7, NLK_CUBIC>(rks_launch);
+extern template __global__ void rks_kernel<double, float, 8, NLK_CUBIC>(rks_launch);
+extern template __global__ void rks_kernel<double, float, 9, NLK_CUBIC>(rks_launch);
+extern template __global__ void rks_kernel<double, float, 10, NLK_CUBIC>(rks_launch);
+extern template __global__ void rks_kernel<double, float, 11, NLK_CUBIC>(rks_launch);
+extern template __global__ void rks_kernel<double, float, 12, NLK_CUBIC>(rks_launch);
+extern template __global__ void rks_kernel<double, float, 13, NLK_CUBIC>(rks_launch);
+
 // ===========================================================================
 // host side: plans and the C ABI
 // ===========================================================================
@@ -161,36 +183,39 @@ struct KernelEntry {
   int lds;
 };
 
-template <typename R, int LOG2N, int NLK>
+template <typename R, typename RN, int LOG2N, int NLK>
 KernelEntry entry_of() {
   KernelEntry e;
-  e.fn = (const void*)&rks_kernel<R, LOG2N, NLK>;
+  e.fn = (const void*)&rks_kernel<R, RN, LOG2N, NLK>;
   e.threads = rks_threads(LOG2N, NLK);
   e.lds = rks_lds_bytes<R, LOG2N>();
   return e;
 }
 
-template <typename R, int NLK>
+template <typename R, typename RN, int NLK>
 bool lookup_nlk(int log2n, KernelEntry* e) {
   switch (log2n) {
-    case 4: *e = entry_of<R, 4, NLK>(); return true;
-    case 5: *e = entry_of<R, 5, NLK>(); return true;
-    case 6: *e = entry_of<R, 6, NLK>(); return true;
-    case 7: *e = entry_of<R, 7, NLK>(); return true;
-    case 8: *e = entry_of<R, 8, NLK>(); return true;
-    case 9: *e = entry_of<R, 9, NLK>(); return true;
-    case 10: *e = entry_of<R, 10, NLK>(); return true;
-    case 11: *e = entry_of<R, 11, NLK>(); return true;
-    case 12: *e = entry_of<R, 12, NLK>(); return true;
-    case 13: *e = entry_of<R, 13, NLK>(); return true;
+    case 4: *e = entry_of<R, RN, 4, NLK>(); return true;
+    case 5: *e = entry_of<R, RN, 5, NLK>(); return true;
+    case 6: *e = entry_of<R, RN, 6, NLK>(); return true;
+    case 7: *e = entry_of<R, RN, 7, NLK>(); return true;
+    case 8: *e = entry_of<R, RN, 8, NLK>(); return true;
+    case 9: *e = entry_of<R, RN, 9, NLK>(); return true;
+    case 10: *e = entry_of<R, RN, 10, NLK>(); return true;
+    case 11: *e = entry_of<R, RN, 11, NLK>(); return true;
+    case 12: *e = entry_of<R, RN, 12, NLK>(); return true;
+    case 13: *e = entry_of<R, RN, 13, NLK>(); return true;
     default: return false;
   }
 }
 
 bool lookup(int prec, int nlk, int log2n, KernelEntry* e) {
-  if (prec == RKS_F64) return nlk == NLK_ADVECT ? lookup_nlk<double, NLK_ADVECT>(log2n, e)
-                                                : lookup_nlk<double, NLK_CUBIC>(log2n, e);
-  return nlk == NLK_ADVECT ? lookup_nlk<float, NLK_ADVECT>(log2n, e) : lookup_nlk<float, NLK_CUBIC>(log2n, e);
+  if (prec == RKS_F64) return nlk == NLK_ADVECT ? lookup_nlk<double, double, NLK_ADVECT>(log2n, e)
+                                                : lookup_nlk<double, double, NLK_CUBIC>(log2n, e);
+  if (prec == RKS_MIXED) return nlk == NLK_ADVECT ? lookup_nlk<double, float, NLK_ADVECT>(log2n, e)
+                                                  : lookup_nlk<double, float, NLK_CUBIC>(log2n, e);
+  return nlk == NLK_ADVECT ? lookup_nlk<float, float, NLK_ADVECT>(log2n, e)
+                           : lookup_nlk<float, float, NLK_CUBIC>(log2n, e);
 }
 
 int ilog2(int n) {
@@ -202,7 +227,10 @@ int ilog2(int n) {
 }  // namespace
 
 struct rks_plan {
-  int method = 0, nlk = 0, prec = 0, ninst = 0, device = 0, esize = 16, cu = 256;
+  int method = 0, nlk = 0, prec = 0, ninst = 0, device = 0, cu = 256;
+  int esize = 16;  // state element bytes (complex)
+  int nsize = 16;  // NL / coefficient storage element bytes (complex)
+  size_t coef_bytes_per_mode = 0;
   rks_tableau tab;
   rks_ctrl ctl;
   std::vector<rks_inst> h_inst;
@@ -256,7 +284,10 @@ int rks_plan_create(int method, int nl_kind, int precision, const rks_config* cf
                     const rks_instance_desc* desc, rks_plan** out) {
   if (!out || !cfg || !desc || ninst <= 0) return fail(RKS_ERR_INVALID, "null argument or ninst <= 0");
   *out = nullptr;
-  if (precision != RKS_F64 && precision != RKS_F32) return fail(RKS_ERR_INVALID, "precision must be F64 or F32");
+  if (precision != RKS_F64 && precision != RKS_F32 && precision != RKS_MIXED)
+    return fail(RKS_ERR_INVALID, "precision must be RKS_F64, RKS_F32 or RKS_MIXED");
+  if (precision == RKS_MIXED && nl_kind == RKS_NL_NONE)
+    return fail(RKS_ERR_INVALID, "mixed precision needs a built-in nonlinearity");
   if (nl_kind != RKS_NL_ADVECT_RFFT && nl_kind != RKS_NL_CUBIC_FFT && nl_kind != RKS_NL_NONE)
     return fail(RKS_ERR_INVALID, "unknown nonlinear kind");
   rks_plan* p = new rks_plan();
@@ -268,7 +299,23 @@ int rks_plan_create(int method, int nl_kind, int precision, const rks_config* cf
   p->nlk = nl_kind;
   p->prec = precision;
   p->ninst = ninst;
-  p->esize = precision == RKS_F64 ? 16 : 8;
+  p->esize = precision == RKS_F32 ? 8 : 16;
+  p->nsize = precision == RKS_F64 ? 16 : 8;
+  // coefficient layout: index order for uniform precision; propagators (state
+  // precision) first, then the rest, for mixed precision
+  {
+    int n_rs = 0;
+    for (int c = 0; c < p->tab.ncoef; ++c) n_rs += p->tab.coef_rs[c] ? 1 : 0;
+    int rank_rs = 0, rank_rn = 0;
+    for (int c = 0; c < p->tab.ncoef; ++c) {
+      if (precision != RKS_MIXED) p->tab.coef_off[c] = c * p->esize;
+      else if (p->tab.coef_rs[c]) p->tab.coef_off[c] = 16 * rank_rs++;
+      else p->tab.coef_off[c] = 16 * n_rs + 8 * rank_rn++;
+      if (precision != RKS_MIXED) p->tab.coef_rs[c] = 1;
+    }
+    p->coef_bytes_per_mode = precision == RKS_MIXED ? (size_t)(16 * n_rs + 8 * (p->tab.ncoef - n_rs))
+                                                    : (size_t)p->tab.ncoef * p->esize;
+  }
   fill_ctrl(&p->ctl, cfg);
   if (cfg->contour_points < 2) {
     delete p;
@@ -316,7 +363,8 @@ int rks_plan_create(int method, int nl_kind, int precision, const rks_config* cf
     }
     p->modes[i] = m;
     p->ws_off[i] = ws_total;
-    ws_total += (size_t)(p->tab.nslot + 2 + p->tab.ncoef) * m;
+    ws_total += rks_align256((size_t)p->tab.nslot * m * p->nsize) + 2 * rks_align256((size_t)m * p->esize) +
+                rks_align256(p->coef_bytes_per_mode * m);
     Loff[i] = L_total;
     L_total += (size_t)m * (desc[i].L_is_complex ? 2 : 1);
     kxoff[i] = kx_total;
@@ -326,14 +374,14 @@ int rks_plan_create(int method, int nl_kind, int precision, const rks_config* cf
   HIPCHK(hipMalloc(&p->d_ctl, sizeof(rks_ctrl)));
   HIPCHK(hipMalloc(&p->d_inst, sizeof(rks_inst) * ninst));
   HIPCHK(hipMalloc(&p->d_io, sizeof(rks_io) * ninst));
-  HIPCHK(hipMalloc(&p->d_ws, ws_total * p->esize));
+  HIPCHK(hipMalloc(&p->d_ws, ws_total));
   HIPCHK(hipMalloc(&p->d_L, L_total * sizeof(double)));
   HIPCHK(hipMalloc(&p->d_kx, kx_total * (p->esize / 2)));
   HIPCHK(hipMalloc(&p->d_queue, sizeof(int) * 64));
   HIPCHK(hipMalloc(&p->d_order, sizeof(int) * ninst));
   HIPCHK(hipMemcpy(p->d_tab, &p->tab, sizeof(rks_tableau), hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(p->d_ctl, &p->ctl, sizeof(rks_ctrl), hipMemcpyHostToDevice));
-  HIPCHK(hipMemset(p->d_ws, 0, ws_total * p->esize));
+  HIPCHK(hipMemset(p->d_ws, 0, ws_total));
   double* d_kx64 = nullptr;
   if (kx_total) HIPCHK(hipMalloc(&d_kx64, kx_total * sizeof(double)));
   for (int i = 0; i < ninst; ++i) {
@@ -343,7 +391,7 @@ int rks_plan_create(int method, int nl_kind, int precision, const rks_config* cf
     if (desc[i].kx) HIPCHK(hipMemcpy(d_kx64 + kxoff[i], desc[i].kx, sizeof(double) * m, hipMemcpyDefault));
     else HIPCHK(hipMemset(d_kx64 + kxoff[i], 0, sizeof(double) * m));
   }
-  if (precision == RKS_F64) {
+  if (precision != RKS_F32) {
     HIPCHK(hipMemcpy(p->d_kx, d_kx64, kx_total * sizeof(double), hipMemcpyDeviceToDevice));
   } else if (kx_total) {
     hipLaunchKernelGGL(rks_f64_to_f32, dim3(256), dim3(256), 0, 0, d_kx64, (float*)p->d_kx, (int)kx_total);
@@ -363,7 +411,7 @@ int rks_plan_create(int method, int nl_kind, int precision, const rks_config* cf
     r.L = p->d_L + Loff[i];
     r.kx = (char*)p->d_kx + kxoff[i] * (p->esize / 2);
     r.nl_coef = desc[i].nl_coef;
-    r.ws = (char*)p->d_ws + p->ws_off[i] * p->esize;
+    r.ws = (char*)p->d_ws + p->ws_off[i];
     r.h_coeff = std::nan("");
     r.ubuf = -1;
     for (int s = 0; s < RKS_MAX_SLOTS; ++s) r.slot_map[s] = s;
@@ -387,7 +435,7 @@ int rks_plan_create(int method, int nl_kind, int precision, const rks_config* cf
       HIPCHK(hipFuncSetAttribute(e.fn, hipFuncAttributeMaxDynamicSharedMemorySize, e.lds));
       const int N = 1 << l;
       HIPCHK(hipMalloc(&p->d_tw[l], (size_t)N * p->esize));
-      if (precision == RKS_F64)
+      if (precision != RKS_F32)
         hipLaunchKernelGGL(rks_twiddle_kernel<double>, dim3((N + 255) / 256), dim3(256), 0, 0,
                            (cx<double>*)p->d_tw[l], N);
       else
@@ -491,6 +539,13 @@ int rks_set_storage(rks_plan* p, int store_freq, int snap_cap, int log_cap) {
   p->snap_cap = snap_cap;
   p->log_cap = log_cap;
   return RKS_OK;
+}
+
+// start of the coefficient region of instance `inst` (mirror of run_instance)
+static char* inst_cbase(const rks_plan* p, int inst) {
+  const size_t M = (size_t)p->modes[inst];
+  return (char*)p->h_inst[inst].ws + rks_align256((size_t)p->tab.nslot * M * p->nsize) +
+         2 * rks_align256(M * p->esize);
 }
 
 static int launch_all(rks_plan* p, int mode, double h_mode, hipStream_t stream) {
@@ -600,25 +655,29 @@ int rks_nl_eval(rks_plan* p, const void* const* u, void* const* nl_out, hipStrea
 
 int rks_coeffs(rks_plan* p, int inst, double h, void* dst, hipStream_t stream) {
   if (!p || inst < 0 || inst >= p->ninst) return fail(RKS_ERR_INVALID, "bad plan or instance");
+  if (p->prec == RKS_MIXED && dst) return fail(RKS_ERR_INVALID, "coefficient copy-out needs uniform precision");
   const int M = p->modes[inst];
   const size_t nbytes = (size_t)p->tab.ncoef * M * p->esize;
   const rks_inst& r = p->h_inst[inst];
-  void* coefs = (char*)r.ws + (size_t)(p->tab.nslot + 2) * M * p->esize;
+  char* cbase = inst_cbase(p, inst);
   const int grid = (M + 255) / 256;
   if (p->prec == RKS_F64)
     hipLaunchKernelGGL(rks_coef_kernel<double>, dim3(grid), dim3(256), 0, stream, p->d_tab, p->d_ctl, r.L,
-                       r.L_complex, M, h, (cx<double>*)coefs);
-  else
+                       r.L_complex, M, h, cbase);
+  else if (p->prec == RKS_F32)
     hipLaunchKernelGGL(rks_coef_kernel<float>, dim3(grid), dim3(256), 0, stream, p->d_tab, p->d_ctl, r.L,
-                       r.L_complex, M, h, (cx<float>*)coefs);
+                       r.L_complex, M, h, cbase);
+  else
+    return fail(RKS_ERR_INVALID, "mixed-precision coefficients are generated inside the stepping kernel");
   HIPCHK(hipGetLastError());
-  if (dst) HIPCHK(hipMemcpyAsync(dst, coefs, nbytes, hipMemcpyDeviceToDevice, stream));
+  if (dst) HIPCHK(hipMemcpyAsync(dst, cbase, nbytes, hipMemcpyDeviceToDevice, stream));
   return RKS_OK;
 }
 
 int rks_combine(rks_plan* p, int inst, int stage, double h, const void* u, const void* const* slots, void* out,
                 hipStream_t stream) {
   if (!p || inst < 0 || inst >= p->ninst) return fail(RKS_ERR_INVALID, "bad plan or instance");
+  if (p->prec == RKS_MIXED) return fail(RKS_ERR_INVALID, "generic path needs uniform precision");
   const rks_tableau& tb = p->tab;
   const rks_term* terms;
   int nterm;
@@ -645,7 +704,7 @@ int rks_combine(rks_plan* p, int inst, int stage, double h, const void* u, const
   const rks_inst& r = p->h_inst[inst];
   A.tab = p->d_tab;
   A.u = u;
-  A.coefs = (char*)r.ws + (size_t)(tb.nslot + 2) * M * p->esize;
+  A.coefs = inst_cbase(p, inst);
   A.out = out;
   A.stage = stage;
   A.is_err = is_err ? 1 : 0;

@@ -1,13 +1,13 @@
-// Persistent stepping kernels, f64 engine precision, cubic nonlinearity, n = 2^4 .. 2^13.
+// Persistent stepping kernels: state/FFT double, NL and coefficient storage double, cubic nonlinearity, n = 2^4 .. 2^13.
 #include "rks_kernels.h"
 
-template __global__ void rks_kernel<double, 4, NLK_CUBIC>(rks_launch);
-template __global__ void rks_kernel<double, 5, NLK_CUBIC>(rks_launch);
-template __global__ void rks_kernel<double, 6, NLK_CUBIC>(rks_launch);
-template __global__ void rks_kernel<double, 7, NLK_CUBIC>(rks_launch);
-template __global__ void rks_kernel<double, 8, NLK_CUBIC>(rks_launch);
-template __global__ void rks_kernel<double, 9, NLK_CUBIC>(rks_launch);
-template __global__ void rks_kernel<double, 10, NLK_CUBIC>(rks_launch);
-template __global__ void rks_kernel<double, 11, NLK_CUBIC>(rks_launch);
-template __global__ void rks_kernel<double, 12, NLK_CUBIC>(rks_launch);
-template __global__ void rks_kernel<double, 13, NLK_CUBIC>(rks_launch);
+template __global__ void rks_kernel<double, double, 4, NLK_CUBIC>(rks_launch);
+template __global__ void rks_kernel<double, double, 5, NLK_CUBIC>(rks_launch);
+template __global__ void rks_kernel<double, double, 6, NLK_CUBIC>(rks_launch);
+template __global__ void rks_kernel<double, double, 7, NLK_CUBIC>(rks_launch);
+template __global__ void rks_kernel<double, double, 8, NLK_CUBIC>(rks_launch);
+template __global__ void rks_kernel<double, double, 9, NLK_CUBIC>(rks_launch);
+template __global__ void rks_kernel<double, double, 10, NLK_CUBIC>(rks_launch);
+template __global__ void rks_kernel<double, double, 11, NLK_CUBIC>(rks_launch);
+template __global__ void rks_kernel<double, double, 12, NLK_CUBIC>(rks_launch);
+template __global__ void rks_kernel<double, double, 13, NLK_CUBIC>(rks_launch);

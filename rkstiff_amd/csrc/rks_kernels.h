@@ -28,6 +28,8 @@
 #define NLK_CUBIC 1
 #define NLK_NONE 2
 
+__host__ __device__ constexpr size_t rks_align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
 struct rks_launch {
   const rks_tableau* tab;
   const rks_ctrl* ctl;
@@ -154,10 +156,11 @@ __device__ __forceinline__ void frac_contour_add(const rks_frac& fr, z64 Z, z64 
   acc[3] = mkc(acc[3].re + p3.re, acc[3].im + p3.im);
 }
 
-// Every coefficient recipe of fraction f: c = (outer * sum_k w_k B_k) / den.
-template <typename R>
+// Every coefficient recipe of fraction f: c = (outer * sum_k w_k B_k) / den,
+// stored at cbase + coef_off[c]*M in state (RS) or storage (RN) precision.
+template <typename RS, typename RN>
 __device__ __forceinline__ void frac_coefs(const rks_tableau& tb, int f, const z64 (&v)[4], double h, int cplx,
-                                           int m, size_t M, cx<R>* __restrict__ coefs) {
+                                           int m, size_t M, char* __restrict__ cbase) {
 #pragma clang fp contract(off)
   for (int c = 0; c < tb.ncoef; ++c) {
     const rks_coef& cf = tb.coef[c];
@@ -175,14 +178,16 @@ __device__ __forceinline__ void frac_coefs(const rks_tableau& tb, int f, const z
       if (cplx) acc = np_rdiv(acc, cf.den);
       else acc = mkc(acc.re / cf.den, acc.im / cf.den);
     }
-    coefs[(size_t)c * M + m] = mkc((R)acc.re, (R)acc.im);
+    char* p = cbase + (size_t)tb.coef_off[c] * M;
+    if (tb.coef_rs[c]) ((cx<RS>*)p)[m] = mkc((RS)acc.re, (RS)acc.im);
+    else ((cx<RN>*)p)[m] = mkc((RN)acc.re, (RN)acc.im);
   }
 }
 
 // All coefficients of one mode by one thread (generic-path kernel).
-template <typename R>
+template <typename RS, typename RN>
 __device__ void coef_mode(const rks_tableau& tb, const rks_ctrl& ct, const double* __restrict__ L,
-                          int L_complex, int m, int M, double h, cx<R>* __restrict__ coefs) {
+                          int L_complex, int m, int M, double h, char* __restrict__ cbase) {
 #pragma clang fp contract(off)
   const int cplx = L_complex || tb.uses_contour;
   const z64 z = mode_z(L, L_complex, m, h);
@@ -207,7 +212,7 @@ __device__ void coef_mode(const rks_tableau& tb, const rks_ctrl& ct, const doubl
         else v[3] = b;
       }
     }
-    frac_coefs<R>(tb, f, v, h, cplx, m, (size_t)M, coefs);
+    frac_coefs<RS, RN>(tb, f, v, h, cplx, m, (size_t)M, cbase);
   }
 }
 
@@ -215,9 +220,10 @@ __device__ void coef_mode(const rks_tableau& tb, const rks_ctrl& ct, const doubl
 // then the small modes' contour means spread over groups of up to 32 lanes
 // (one lane per node, deterministic shuffle reduction).  `list` is LDS
 // scratch for up to M ints; *count is an LDS counter.
-template <typename R, int T, int MPT>
-__device__ __attribute__((noinline)) void coef_update(const rks_tableau& tb, const rks_ctrl& ct, const double* __restrict__ L,
-                            int L_complex, int M, double h, cx<R>* __restrict__ coefs, int* list, int* count) {
+template <typename RS, typename RN, int T, int MPT>
+__device__ __attribute__((noinline)) void coef_update(const rks_tableau& tb, const rks_ctrl& ct,
+                                                      const double* __restrict__ L, int L_complex, int M, double h,
+                                                      char* __restrict__ cbase, int* list, int* count) {
 #pragma clang fp contract(off)
   const int t = threadIdx.x;
   const int cplx = L_complex || tb.uses_contour;
@@ -235,7 +241,7 @@ __device__ __attribute__((noinline)) void coef_update(const rks_tableau& tb, con
       z64 v[4];
       v[1] = v[2] = v[3] = mkc(0.0, 0.0);
       frac_direct(tb.frac[f], z, h, cplx, v);
-      frac_coefs<R>(tb, f, v, h, cplx, m, (size_t)M, coefs);
+      frac_coefs<RS, RN>(tb, f, v, h, cplx, m, (size_t)M, cbase);
     }
   }
   __syncthreads();
@@ -274,7 +280,7 @@ __device__ __attribute__((noinline)) void coef_update(const rks_tableau& tb, con
           }
           if (lane == 0) {
             v[0] = np_exp(frac_arg(z, fr.fnum, fr.fden, cplx));
-            frac_coefs<R>(tb, f, v, h, cplx, m, (size_t)M, coefs);
+            frac_coefs<RS, RN>(tb, f, v, h, cplx, m, (size_t)M, cbase);
           }
         }
       }
@@ -284,156 +290,113 @@ __device__ __attribute__((noinline)) void coef_update(const rks_tableau& tb, con
 }
 
 // ---------------------------------------------------------------------------
-// one term of a stage combination / error estimate (grouping of rks_term)
-// ---------------------------------------------------------------------------
-template <typename R, typename SlotF>
-__device__ __forceinline__ cx<R> term_value(const rks_term& tm, SlotF slotp, const cx<R>* __restrict__ coefs,
-                                            size_t M, int m, double h) {
-  using C = cx<R>;
-  C v = slotp(tm.src)[m];
-  if (tm.src2 >= 0) v = v + (R)tm.scale2 * slotp(tm.src2)[m];
-  C c;
-  if (tm.hmode == 0) {
-    c = (tm.coef < 0) ? v : coefs[(size_t)tm.coef * M + m] * v;
-    if (tm.scale != 1.0) c = (R)tm.scale * c;
-  } else if (tm.hmode == 1) {
-    const R hr = (R)h;
-    c = (tm.coef < 0) ? hr * v : (hr * coefs[(size_t)tm.coef * M + m]) * v;
-    if (tm.scale != 1.0) c = (R)tm.scale * c;
-  } else {
-    c = (R)((tm.scale * h) / tm.den) * v;
-  }
-  return c;
-}
-
-// k = E u + terms (or E u + h * sum(terms) when the stage factors h out)
-template <typename R, typename SlotF>
-__device__ __forceinline__ cx<R> stage_value(const rks_stage& S, const cx<R>* __restrict__ u, SlotF slotp,
-                                             const cx<R>* __restrict__ coefs, size_t M, int m, double h) {
-  using C = cx<R>;
-  C acc = (S.ecoef < 0) ? u[m] : coefs[(size_t)S.ecoef * M + m] * u[m];
-  if (S.hsum) {
-    C sum = term_value<R>(S.t[0], slotp, coefs, M, m, h);
-    for (int j = 1; j < S.nterm; ++j) sum = sum + term_value<R>(S.t[j], slotp, coefs, M, m, h);
-    acc = acc + (R)h * sum;
-  } else {
-    for (int j = 0; j < S.nterm; ++j) acc = acc + term_value<R>(S.t[j], slotp, coefs, M, m, h);
-  }
-  return acc;
-}
-
-// Chunked, term-major forms of stage_value / error_value: CH modes
+// Stage combinations and the error estimate, chunked and term-major: CH modes
 // m = m0 + i*T of one thread at once, so every term issues CH independent
-// loads (the per-mode forms above wait on one mode at a time).
-template <typename R, int CH, typename SlotF>
-__device__ __forceinline__ void stage_chunk(const rks_stage& S, const cx<R>* __restrict__ u, SlotF slotp,
-                                            const cx<R>* __restrict__ coefs, size_t M, int m0, int T, int mlim,
-                                            double h, cx<R> (&acc)[CH]) {
-  using C = cx<R>;
-  const C* ec = S.ecoef < 0 ? nullptr : coefs + (size_t)S.ecoef * M;
+// loads.  Arithmetic is in the state precision RS; NL arrays and the non-
+// propagator coefficients may be stored in RN (converted on load).
+//   term (rks_term grouping):  inner = NL[src] + scale2 * NL[src2]
+//     hmode 0: scale * (C * inner)   1: ((h * C) * inner) * scale   2: ((scale*h)/den) * inner
+// ---------------------------------------------------------------------------
+template <typename RS, typename RN>
+struct CoefPtr {
+  const cx<RS>* s;
+  const cx<RN>* n;
+  __device__ __forceinline__ cx<RS> operator[](int m) const { return s ? s[m] : cvt<RS>(n[m]); }
+};
+
+template <typename RS, typename RN>
+__device__ __forceinline__ CoefPtr<RS, RN> coef_ptr(const rks_tableau& tb, const char* cbase, size_t M, int c) {
+  CoefPtr<RS, RN> p;
+  const char* q = cbase + (size_t)tb.coef_off[c] * M;
+  p.s = tb.coef_rs[c] ? (const cx<RS>*)q : nullptr;
+  p.n = tb.coef_rs[c] ? nullptr : (const cx<RN>*)q;
+  return p;
+}
+
+// slotp(s): NL array of slot s (const cx<RN>*)
+template <typename RS, typename RN, int CH, bool HSUM, typename SlotF>
+__device__ __forceinline__ void add_terms(const rks_tableau& tb, const rks_term* terms, int nterm, SlotF slotp,
+                                          const char* cbase, size_t M, int m0, int T, int mlim, double h,
+                                          cx<RS> (&acc)[CH]) {
+  using C = cx<RS>;
+  for (int j = 0; j < nterm; ++j) {
+    const rks_term& tm = terms[j];
+    const cx<RN>* nl = slotp(tm.src);
+    const cx<RN>* nl2 = tm.src2 >= 0 ? slotp(tm.src2) : nullptr;
+    const bool hc = tm.coef >= 0;
+    const CoefPtr<RS, RN> cf = hc ? coef_ptr<RS, RN>(tb, cbase, M, tm.coef) : CoefPtr<RS, RN>{nullptr, nullptr};
+    const RS s2 = (RS)tm.scale2;
+    const RS hr = (RS)h;
+    const RS sc = (RS)tm.scale;
+    const RS hs = (RS)((tm.scale * h) / tm.den);
+    const bool unit = tm.scale == 1.0;
+#pragma unroll
+    for (int i = 0; i < CH; ++i) {
+      const int m = m0 + i * T;
+      if (m < mlim) {
+        C v = cvt<RS>(nl[m]);
+        if (nl2) v = v + s2 * cvt<RS>(nl2[m]);
+        C c;
+        if (tm.hmode == 0) {
+          c = hc ? cf[m] * v : v;
+          if (!unit) c = sc * c;
+        } else if (tm.hmode == 1) {
+          c = hc ? (hr * cf[m]) * v : hr * v;
+          if (!unit) c = sc * c;
+        } else {
+          c = hs * v;
+        }
+        acc[i] = (HSUM && j == 0) ? c : acc[i] + c;
+      }
+    }
+  }
+}
+
+// k = E u + terms, or E u + h * sum(terms) for stages that factor h out (if34.py:126-128)
+template <typename RS, typename RN, int CH, typename SlotF>
+__device__ __forceinline__ void stage_chunk(const rks_tableau& tb, const rks_stage& S, const cx<RS>* __restrict__ u,
+                                            SlotF slotp, const char* cbase, size_t M, int m0, int T, int mlim,
+                                            double h, cx<RS> (&acc)[CH]) {
+  using C = cx<RS>;
+  const bool he = S.ecoef >= 0;
+  const CoefPtr<RS, RN> ec = he ? coef_ptr<RS, RN>(tb, cbase, M, S.ecoef) : CoefPtr<RS, RN>{nullptr, nullptr};
 #pragma unroll
   for (int i = 0; i < CH; ++i) {
     const int m = m0 + i * T;
-    acc[i] = mkc((R)0, (R)0);
-    if (m < mlim) acc[i] = ec ? ec[m] * u[m] : u[m];
-  }
-  C sum[CH];
-#pragma unroll
-  for (int i = 0; i < CH; ++i) sum[i] = mkc((R)0, (R)0);
-  for (int j = 0; j < S.nterm; ++j) {
-    const rks_term& tm = S.t[j];
-    const C* nl = slotp(tm.src);
-    const C* nl2 = tm.src2 >= 0 ? slotp(tm.src2) : nullptr;
-    const C* cf = tm.coef >= 0 ? coefs + (size_t)tm.coef * M : nullptr;
-    const R s2 = (R)tm.scale2;
-    const R hr = (R)h;
-    const R sc = (R)tm.scale;
-    const R hs = (R)((tm.scale * h) / tm.den);
-#pragma unroll
-    for (int i = 0; i < CH; ++i) {
-      const int m = m0 + i * T;
-      if (m < mlim) {
-        C v = nl[m];
-        if (nl2) v = v + s2 * nl2[m];
-        C c;
-        if (tm.hmode == 0) {
-          c = cf ? cf[m] * v : v;
-          if (tm.scale != 1.0) c = sc * c;
-        } else if (tm.hmode == 1) {
-          c = cf ? (hr * cf[m]) * v : hr * v;
-          if (tm.scale != 1.0) c = sc * c;
-        } else {
-          c = hs * v;
-        }
-        if (S.hsum) sum[i] = (j == 0) ? c : sum[i] + c;
-        else acc[i] = acc[i] + c;
-      }
-    }
+    acc[i] = mkc((RS)0, (RS)0);
+    if (m < mlim) acc[i] = he ? ec[m] * u[m] : u[m];
   }
   if (S.hsum) {
+    C sum[CH];
+    add_terms<RS, RN, CH, true>(tb, S.t, S.nterm, slotp, cbase, M, m0, T, mlim, h, sum);
 #pragma unroll
-    for (int i = 0; i < CH; ++i) acc[i] = acc[i] + (R)h * sum[i];
+    for (int i = 0; i < CH; ++i)
+      if (m0 + i * T < mlim) acc[i] = acc[i] + (RS)h * sum[i];
+  } else {
+    add_terms<RS, RN, CH, false>(tb, S.t, S.nterm, slotp, cbase, M, m0, T, mlim, h, acc);
   }
 }
 
-template <typename R, int CH, typename SlotF>
-__device__ __forceinline__ void error_chunk(const rks_tableau& tb, SlotF slotp, const cx<R>* __restrict__ coefs,
-                                            size_t M, int m0, int T, int mlim, double h, cx<R> (&ev)[CH]) {
-  using C = cx<R>;
+// embedded error estimate (etd35.py:344, etd34.py:190, if34.py:130, if45dp.py:170-177)
+template <typename RS, typename RN, int CH, typename SlotF>
+__device__ __forceinline__ void error_chunk(const rks_tableau& tb, SlotF slotp, const char* cbase, size_t M, int m0,
+                                            int T, int mlim, double h, cx<RS> (&ev)[CH]) {
 #pragma unroll
-  for (int i = 0; i < CH; ++i) ev[i] = mkc((R)0, (R)0);
-  for (int j = 0; j < tb.nerr; ++j) {
-    const rks_term& tm = tb.err[j];
-    const C* nl = slotp(tm.src);
-    const C* nl2 = tm.src2 >= 0 ? slotp(tm.src2) : nullptr;
-    const C* cf = tm.coef >= 0 ? coefs + (size_t)tm.coef * M : nullptr;
-    const R s2 = (R)tm.scale2;
-    const R sc = (R)tm.scale;
-    const R hs = (R)((tm.scale * h) / tm.den);
+  for (int i = 0; i < CH; ++i) ev[i] = mkc((RS)0, (RS)0);
+  add_terms<RS, RN, CH, true>(tb, tb.err, tb.nerr, slotp, cbase, M, m0, T, mlim, h, ev);
+  if (tb.err_factored) {
+    const bool hc = tb.err_coef >= 0;
+    const CoefPtr<RS, RN> cf = hc ? coef_ptr<RS, RN>(tb, cbase, M, tb.err_coef) : CoefPtr<RS, RN>{nullptr, nullptr};
 #pragma unroll
     for (int i = 0; i < CH; ++i) {
       const int m = m0 + i * T;
       if (m < mlim) {
-        C v = nl[m];
-        if (nl2) v = v + s2 * nl2[m];
-        C c;
-        if (tm.hmode == 2) {
-          c = hs * v;
-        } else {
-          c = cf ? cf[m] * v : v;
-          if (tm.scale != 1.0) c = sc * c;
-        }
-        ev[i] = (j == 0) ? c : ev[i] + c;
+        if (tb.err_hpow) ev[i] = (RS)h * ev[i];
+        if (tb.err_scale != 1.0) ev[i] = (RS)tb.err_scale * ev[i];
+        if (hc) ev[i] = cf[m] * ev[i];
       }
     }
   }
-  if (tb.err_factored) {
-    const C* cf = tb.err_coef >= 0 ? coefs + (size_t)tb.err_coef * M : nullptr;
-#pragma unroll
-    for (int i = 0; i < CH; ++i) {
-      const int m = m0 + i * T;
-      if (m < mlim) {
-        if (tb.err_hpow) ev[i] = (R)h * ev[i];
-        if (tb.err_scale != 1.0) ev[i] = (R)tb.err_scale * ev[i];
-        if (cf) ev[i] = cf[m] * ev[i];
-      }
-    }
-  }
-}
-
-// embedded error estimate of the tableau (etd35.py:344, if34.py:130, if45dp.py:170-177)
-template <typename R, typename SlotF>
-__device__ __forceinline__ cx<R> error_value(const rks_tableau& tb, SlotF slotp, const cx<R>* __restrict__ coefs,
-                                             size_t M, int m, double h) {
-  using C = cx<R>;
-  C ev = term_value<R>(tb.err[0], slotp, coefs, M, m, h);
-  for (int j = 1; j < tb.nerr; ++j) ev = ev + term_value<R>(tb.err[j], slotp, coefs, M, m, h);
-  if (tb.err_factored) {
-    if (tb.err_hpow) ev = (R)h * ev;
-    if (tb.err_scale != 1.0) ev = (R)tb.err_scale * ev;
-    if (tb.err_coef >= 0) ev = coefs[(size_t)tb.err_coef * M + m] * ev;
-  }
-  return ev;
 }
 
 // ---------------------------------------------------------------------------
@@ -489,8 +452,8 @@ __device__ __forceinline__ void fill_nl(GetK getk, cx<R>* buf, const R* __restri
   }
 }
 
-template <typename R, int LOG2N, int T, int NLK>
-__device__ __attribute__((noinline)) void nl_core(cx<R>* buf, const cx<R>* twl, R coef, cx<R>* __restrict__ dst) {
+template <typename R, typename RN, int LOG2N, int T, int NLK>
+__device__ __attribute__((noinline)) void nl_core(cx<R>* buf, const cx<R>* twl, R coef, cx<RN>* __restrict__ dst) {
   using C = cx<R>;
   using rksfft::pad;
   constexpr int N = 1 << LOG2N;
@@ -512,7 +475,7 @@ __device__ __attribute__((noinline)) void nl_core(cx<R>* buf, const cx<R>* twl, 
     for (int i = 0; i < N / T; ++i) {
       const int m = t + i * T;
       const C w = buf[pad(m)];
-      dst[m] = mkc(-coef * w.im, coef * w.re);
+      dst[m] = mkc((RN)(-coef * w.im), (RN)(coef * w.re));
     }
   } else {
     constexpr int M = N / 2 + 1;
@@ -541,27 +504,28 @@ __device__ __attribute__((noinline)) void nl_core(cx<R>* buf, const cx<R>* twl, 
         const C D = Zk - Zc;
         const C O = mkc((R)0.5 * D.im, (R)-0.5 * D.re);
         const C W = E + rksfft::twiddle<R, LOG2N>(twl, m) * O;
-        dst[m] = coef * W;
+        dst[m] = cvt<RN>(coef * W);
       }
     }
   }
   __syncthreads();
 }
 
-template <typename R, int LOG2N, int T, int NLK, typename GetK>
+template <typename R, typename RN, int LOG2N, int T, int NLK, typename GetK>
 __device__ __forceinline__ void eval_nl(GetK getk, cx<R>* buf, const cx<R>* twl, const R* __restrict__ kx, R coef,
-                                        cx<R>* __restrict__ dst) {
+                                        cx<RN>* __restrict__ dst) {
   fill_nl<R, LOG2N, T, NLK>(getk, buf, kx);
-  nl_core<R, LOG2N, T, NLK>(buf, twl, coef, dst);
+  nl_core<R, RN, LOG2N, T, NLK>(buf, twl, coef, dst);
 }
 
 // ---------------------------------------------------------------------------
 // one instance: evolve / step / nl / coefficients
 // ---------------------------------------------------------------------------
-template <typename R, int LOG2N, int T, int NLK>
+template <typename R, typename RN, int LOG2N, int T, int NLK>
 __device__ void run_instance(const rks_launch& P, int idx, cx<R>* buf, const cx<R>* twl, double* red,
                              int* lds_count) {
   using C = cx<R>;
+  using CN = cx<RN>;
   constexpr int N = 1 << LOG2N;
   constexpr int M = (NLK == NLK_ADVECT) ? N / 2 + 1 : N;
   constexpr int MPT = (M + T - 1) / T;
@@ -576,12 +540,14 @@ __device__ void run_instance(const rks_launch& P, int idx, cx<R>* buf, const cx<
   const R nlc = (R)in->nl_coef;
   const double* Ld = in->L;
   const int Lc = in->L_complex;
-  C* ws = (C*)in->ws;
+  // workspace: nslot NL arrays (RN) | 2 candidate states (R) | coefficients
+  // (each region 256-byte aligned; layout computed by the host, rks_ws_layout)
+  char* ws = (char*)in->ws;
   const int nslot = tb.nslot;
-  C* slot_base = ws;
-  C* const cand0 = ws + (size_t)nslot * M;
-  C* const cand1 = ws + (size_t)(nslot + 1) * M;
-  C* coefs = ws + (size_t)(nslot + 2) * M;
+  CN* slot_base = (CN*)ws;
+  C* const cand0 = (C*)(ws + rks_align256((size_t)nslot * M * sizeof(CN)));
+  C* const cand1 = cand0 + rks_align256((size_t)M * sizeof(C)) / sizeof(C);
+  char* cbase = (char*)(cand1 + rks_align256((size_t)M * sizeof(C)) / sizeof(C));
 
   if (P.mode == RKS_MODE_NL) {
     const C* u = (const C*)io.u_in;
@@ -592,13 +558,13 @@ __device__ void run_instance(const rks_launch& P, int idx, cx<R>* buf, const cx<
         acc[i] = m < M ? u[m] : mkc((R)0, (R)0);
       }
     };
-    eval_nl<R, LOG2N, T, NLK>(getu, buf, twl, kx, nlc, (C*)io.u_out);
+    eval_nl<R, R, LOG2N, T, NLK>(getu, buf, twl, kx, nlc, (C*)io.u_out);
     return;
   }
   if (P.mode == RKS_MODE_COEFFS) {
     for (int i = 0; i < MPT; ++i) {
       const int m = t + i * T;
-      if (m < M) coef_mode<R>(tb, ct, Ld, Lc, m, M, P.h_mode, coefs);
+      if (m < M) coef_mode<R, RN>(tb, ct, Ld, Lc, m, M, P.h_mode, cbase);
     }
     return;
   }
@@ -650,7 +616,7 @@ __device__ void run_instance(const rks_launch& P, int idx, cx<R>* buf, const cx<
     attempts = rejects = 0;
   }
 
-  auto slot = [&](int s) -> C* { return slot_base + (size_t)((smap >> (4 * s)) & 15u) * M; };
+  auto slot = [&](int s) -> CN* { return slot_base + (size_t)((smap >> (4 * s)) & 15u) * M; };
   auto cur_u = [&]() -> const C* { return ubuf < 0 ? uin : (const C*)(ubuf == 0 ? cand0 : cand1); };
 
   bool done = false;
@@ -672,7 +638,7 @@ __device__ void run_instance(const rks_launch& P, int idx, cx<R>* buf, const cx<
     // ---------------- attempt loop (solveras.py:384-410) ----------------
     for (;;) {
       if (h != h_coeff) {
-        coef_update<R, T, MPT>(tb, ct, Ld, Lc, M, h, coefs, (int*)buf, lds_count);
+        coef_update<R, RN, T, MPT>(tb, ct, Ld, Lc, M, h, cbase, (int*)buf, lds_count);
         h_coeff = h;
         ++coef_updates;
       }
@@ -684,12 +650,12 @@ __device__ void run_instance(const rks_launch& P, int idx, cx<R>* buf, const cx<
         }
       };
       if (!started) {
-        eval_nl<R, LOG2N, T, NLK>(getu, buf, twl, kx, nlc, slot(0));
+        eval_nl<R, RN, LOG2N, T, NLK>(getu, buf, twl, kx, nlc, slot(0));
         ++nl_calls;
         started = 1;
       } else if (accept || !tb.adaptive) {
         if (tb.nl1_policy == RKS_NL1_RECOMPUTE) {
-          eval_nl<R, LOG2N, T, NLK>(getu, buf, twl, kx, nlc, slot(0));
+          eval_nl<R, RN, LOG2N, T, NLK>(getu, buf, twl, kx, nlc, slot(0));
           ++nl_calls;
         } else {
           const unsigned f4 = 4u * (unsigned)tb.fsal_slot;
@@ -701,7 +667,7 @@ __device__ void run_instance(const rks_launch& P, int idx, cx<R>* buf, const cx<
         const rks_stage& S = tb.st[s];
         const bool last = (s == tb.nstage - 1);
         auto getk = [&](int m0, C(&acc)[Chunk<MPT>::CH]) {
-          stage_chunk<R, Chunk<MPT>::CH>(S, u, slot, coefs, (size_t)M, m0, T, M, h, acc);
+          stage_chunk<R, RN, Chunk<MPT>::CH>(tb, S, u, slot, cbase, (size_t)M, m0, T, M, h, acc);
           if (last) {
 #pragma unroll
             for (int i = 0; i < Chunk<MPT>::CH; ++i)
@@ -709,7 +675,7 @@ __device__ void run_instance(const rks_launch& P, int idx, cx<R>* buf, const cx<
           }
         };
         if (S.has_nl) {
-          eval_nl<R, LOG2N, T, NLK>(getk, buf, twl, kx, nlc, slot(S.nl_slot));
+          eval_nl<R, RN, LOG2N, T, NLK>(getk, buf, twl, kx, nlc, slot(S.nl_slot));
           ++nl_calls;
         } else {
 #pragma unroll 1
@@ -742,7 +708,7 @@ __device__ void run_instance(const rks_launch& P, int idx, cx<R>* buf, const cx<
         constexpr int CH = Chunk<MPT>::CH;
         const int m0 = t + c * CH * T;
         C ev[CH];
-        error_chunk<R, CH>(tb, slot, coefs, (size_t)M, m0, T, M, h, ev);
+        error_chunk<R, RN, CH>(tb, slot, cbase, (size_t)M, m0, T, M, h, ev);
 #pragma unroll
         for (int i = 0; i < CH; ++i) {
           const int m = m0 + i * T;
@@ -864,7 +830,7 @@ __device__ void run_instance(const rks_launch& P, int idx, cx<R>* buf, const cx<
 // coefficient code spills instead of lowering occupancy.
 __host__ __device__ constexpr int rks_min_waves(int log2n) { return log2n >= 13 ? 2 : 3; }
 
-template <typename R, int LOG2N, int NLK>
+template <typename R, typename RN, int LOG2N, int NLK>
 __global__ __launch_bounds__(rks_threads(LOG2N, NLK), rks_min_waves(LOG2N)) void rks_kernel(rks_launch P) {
   constexpr int T = rks_threads(LOG2N, NLK);
   extern __shared__ __align__(16) unsigned char rks_smem[];
@@ -880,7 +846,7 @@ __global__ __launch_bounds__(rks_threads(LOG2N, NLK), rks_min_waves(LOG2N)) void
     const int item = s_item;
     __syncthreads();
     if (item >= P.count) break;
-    run_instance<R, LOG2N, T, NLK>(P, P.order[item], buf, twl, red, &s_count);
+    run_instance<R, RN, LOG2N, T, NLK>(P, P.order[item], buf, twl, red, &s_count);
     __syncthreads();
   }
 }

@@ -103,7 +103,18 @@ struct Builder {
     t.err_hpow = hpow;
   }
   void err(const rks_term& x) { t.err[t.nerr++] = x; }
-  rks_tableau finalize() { return t; }
+  // propagators (coefficients multiplying u) keep the state precision
+  rks_tableau finalize() {
+    for (int c = 0; c < t.ncoef; ++c) t.coef_rs[c] = 0;
+    for (int s = 0; s < t.nstage; ++s)
+      if (t.st[s].ecoef >= 0) t.coef_rs[t.st[s].ecoef] = 1;
+    int off = 0;
+    for (int c = 0; c < t.ncoef; ++c) {
+      t.coef_off[c] = off;
+      off += 16;
+    }
+    return t;
+  }
 };
 
 // ETD35 / ETD5 share the fifth-order coefficient rows (etd35.py:213-237).

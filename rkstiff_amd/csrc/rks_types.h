@@ -97,6 +97,8 @@ typedef struct rks_tableau {
   int ncoef;
   int nfrac;
   int uses_contour;    // ETD methods: small modes through the contour mean
+  int coef_rs[RKS_MAX_COEF];   // 1: stored in state precision (multiplies u in some stage)
+  int coef_off[RKS_MAX_COEF];  // byte offset of coefficient c per mode (x m), set per plan
   int err_coef;        // factored error estimate: err = C[err_coef] * (((h^err_hpow) * sum(terms)) * err_scale)
   int err_factored;    //   (0: err = sum of the terms)
   int err_hpow;

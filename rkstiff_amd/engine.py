@@ -90,7 +90,7 @@ class Plan:
         _lib.check(lib.rks_plan_info(self.h, None, ctypes.byref(nslot), ctypes.byref(ncoef),
                                      ctypes.byref(es)))
         self.nslot, self.ncoef, self.esize = nslot.value, ncoef.value, es.value
-        self.dtype = torch.complex128 if precision == _lib.F64 else torch.complex64
+        self.dtype = torch.complex64 if precision == _lib.F32 else torch.complex128
         self.storage = (0, 0, 0)
 
     def __del__(self):

@@ -225,3 +225,37 @@ def test_kdv_fp32_close_to_fp64(rk, golden, method):
     err = rel(host(uf).astype(np.complex128), g[f"{method}/u_final"])
     print(method, "fp32 vs fp64 rel", err)
     assert err < 1e-4
+
+
+# -- batched ensembles ----------------------------------------------------------
+def test_batch_mixed_sizes_match_single_runs(rk):
+    """KS instances of different n and domain lengths in one launch vs one-at-a-time oracle runs."""
+    from rkstiff_amd.batch import BatchSolver
+    from oracle import problems as P
+    insts = P.ks_ensemble(12, seed=7, sizes=(64, 128, 256, 512))
+    probs = [(torch.as_tensor(p["L"]), rk.AdvectiveNL(torch.as_tensor(p["kx"]), -1.0)) for p in insts]
+    bs = BatchSolver("IF34", probs, precision="f64", config=rk.SolverConfig(epsilon=1e-5))
+    uf = bs.evolve([dev(p["u0"]) for p in insts], 0.0, 5.0)
+    assert not bs.failures()
+    for p, u in zip(insts, uf):
+        drv = solvers.make_solver("IF34", p["L"], p["nl"], solvers.SolverConfig(epsilon=1e-5))
+        ref = drv.evolve(p["u0"], t0=0.0, tf=5.0, store_data=False)
+        assert rel(host(u), ref) < 1e-10, (p["n"], rel(host(u), ref))
+
+
+@pytest.mark.parametrize("method", ["ETD35", "IF45DP"])
+def test_batch_mixed_precision_within_gate(rk, method):
+    """Mixed precision (FP64 state/propagators/FFT, FP32 NL storage) stays within 1e-6 of FP64."""
+    from rkstiff_amd.batch import BatchSolver
+    from oracle import problems as P
+    insts = P.nls_ensemble(6, seed=3, sizes=(512, 1024))
+    probs = [(torch.as_tensor(p["L"]), rk.CubicNL(p["n"], 1.0)) for p in insts]
+    cfg = rk.SolverConfig(epsilon=1e-6)
+    u0 = [dev(p["u0"]) for p in insts]
+    f64 = BatchSolver(method, probs, precision="f64", config=cfg).evolve(u0, 0.0, 3.0)
+    mix_solver = BatchSolver(method, probs, precision="mixed", config=cfg)
+    mix = mix_solver.evolve(u0, 0.0, 3.0)
+    assert not mix_solver.failures()
+    errs = [rel(host(a), host(b)) for a, b in zip(mix, f64)]
+    print(method, "mixed vs f64", max(errs))
+    assert max(errs) < 1e-6
