@@ -47,8 +47,12 @@ enum {
   RKS_NL_ADVECT_RFFT = 0, /* c * rfft(irfft(u) * irfft(i kx u)): KS/Burgers (c=-1, README.md:94-97,
                              models.py:189-192), KdV (c=-6, models.py:140-143); n real points */
   RKS_NL_CUBIC_FFT = 1,   /* i g fft(|v|^2 v), v = ifft(u): cubic NLS (demos/nls.ipynb cell 29); n points */
-  RKS_NL_NONE = 2         /* caller evaluates nl (generic path): only coefficients, stage combinations
+  RKS_NL_NONE = 2,        /* caller evaluates nl (generic path): only coefficients, stage combinations
                              and the error norm run here; n = number of modes */
+  RKS_NL_AC2D_RFFT2 = 3   /* rfft2(w - c w^3), w = irfft2(u): Allen-Cahn reaction term (the cubic of
+                             demos/allen_cahn.ipynb cell 4) on an n x n periodic grid, modes flattened
+                             row-major as the reference requires of multi-D problems (solver.py:129-135);
+                             n a power of two in [32, 512], ny = n, m = n (n/2 + 1) */
 };
 
 /* engine precision: state/FFT/propagators + NL and coefficient storage
@@ -74,7 +78,8 @@ typedef struct rks_instance_desc {
   int L_is_complex;    /* lin_op dtype: 0 float64[m], 1 complex128[m] */
   const void* L;       /* lin_op, float64 data, host or device pointer (copied) */
   const double* kx;    /* wavenumbers float64[m] (RKS_NL_ADVECT_RFFT), host or device (copied) */
-  double nl_coef;      /* c of the advective term / g of the cubic term */
+  double nl_coef;      /* c of the advective term / g of the cubic term / c of the rfft2 term */
+  int ny;              /* rows of a 2-D grid (RKS_NL_AC2D_RFFT2, = n); ignored otherwise */
 } rks_instance_desc;
 
 typedef struct rks_stats {

@@ -7,11 +7,12 @@ terms (``nonlinear``), uniform grids (``grids``) and the batched ensemble
 stepper (``batch.BatchSolver``).  Compute runs in librkstiff_amd.so only.
 """
 from .config import ETDConfig, SolverConfig, SolverType
-from .nonlinear import AdvectiveNL, CubicNL, burgers_ops, kdv_ops, ks_ops, nls_ops
+from .nonlinear import (AdvectiveNL, AllenCahnNL, CubicNL, allen_cahn_ops, burgers_ops, kdv_ops, ks_ops,
+                        nls_ops)
 from .solvers import (ETD4, ETD5, ETD34, ETD35, IF4, IF34, IF45DP, BaseSolverAS, BaseSolverCS)
 
 __all__ = [
     "ETD35", "ETD34", "ETD4", "ETD5", "IF34", "IF45DP", "IF4", "BaseSolverAS", "BaseSolverCS",
-    "SolverConfig", "ETDConfig", "SolverType", "AdvectiveNL", "CubicNL", "ks_ops", "kdv_ops",
-    "burgers_ops", "nls_ops",
+    "SolverConfig", "ETDConfig", "SolverType", "AdvectiveNL", "CubicNL", "AllenCahnNL", "ks_ops",
+    "kdv_ops", "burgers_ops", "nls_ops", "allen_cahn_ops",
 ]

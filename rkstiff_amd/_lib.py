@@ -13,7 +13,7 @@ LIB_PATH = os.path.join(HERE, "librkstiff_amd.so")
 
 RKS_OK, RKS_ERR_INVALID, RKS_ERR_HIP, RKS_MAXLOOPS, RKS_MINSTEP, RKS_PAUSED, RKS_RUNNING = range(7)
 METHOD_IDS = {"ETD35": 0, "ETD34": 1, "ETD4": 2, "ETD5": 3, "IF34": 4, "IF45DP": 5, "IF4": 6}
-NL_ADVECT_RFFT, NL_CUBIC_FFT, NL_NONE = 0, 1, 2
+NL_ADVECT_RFFT, NL_CUBIC_FFT, NL_NONE, NL_AC2D_RFFT2 = 0, 1, 2, 3
 F64, F32, MIXED = 0, 1, 2
 
 EXPORTS = (
@@ -35,7 +35,8 @@ class Config(ctypes.Structure):
 
 class InstanceDesc(ctypes.Structure):
     _fields_ = [("n", ctypes.c_int), ("L_is_complex", ctypes.c_int),
-                ("L", ctypes.c_void_p), ("kx", ctypes.c_void_p), ("nl_coef", ctypes.c_double)]
+                ("L", ctypes.c_void_p), ("kx", ctypes.c_void_p), ("nl_coef", ctypes.c_double),
+                ("ny", ctypes.c_int)]
 
 
 class Stats(ctypes.Structure):
@@ -127,6 +128,9 @@ def load(path=None):
             if name not in ("rks_last_error",):
                 getattr(lib, name).restype = ctypes.c_int
         assert lib.rks_sizeof(0) == ctypes.sizeof(Tableau), "tableau layout mismatch"
+        assert lib.rks_sizeof(4) == ctypes.sizeof(InstanceDesc), "instance descriptor layout mismatch"
+        assert lib.rks_sizeof(5) == ctypes.sizeof(Config), "config layout mismatch"
+        assert lib.rks_sizeof(6) == ctypes.sizeof(Stats), "stats layout mismatch"
         _lib = lib
         return lib
 

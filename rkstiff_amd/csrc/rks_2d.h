@@ -1,0 +1,543 @@
+// Batched engine for 2-D periodic problems on an N x N rfft2 grid.
+//
+// Layout: NumPy's rfft2 order, row-major [ky][kx] with kx = 0..N/2, so an
+// instance has M = N (N/2 + 1) modes; the reference flattens it to a 1-D
+// diagonal lin_op (solver.py:129-135, demos/allen_cahn.ipynb).  Built-in term
+// (Allen-Cahn): nl(u) = rfft2(w - c w^3), w = irfft2(u).
+//
+// One trajectory does not fit one workgroup's LDS (512 x 257 complex), so an
+// attempt is a fixed sequence of kernels over all trajectories, each one pass
+// over HBM:
+//   col   forward column FFTs of the previous stage's transform buffer T
+//         -> NL slot; the next stage combination E u + sum C NL formed in
+//         registers; inverse column FFTs -> T (CG adjacent columns per
+//         workgroup, so every row segment read or written is contiguous)
+//   row   per pair of rows: one complex inverse FFT of both rows' half
+//         spectra, w - c w^3 on the real and imaginary parts, one forward
+//         FFT, split into the two rows' half spectra (in place in T)
+//   norm  masked sums of the embedded error norm (solveras.py:449-455),
+//         FP64 partials per workgroup
+//   ctrl  FP64 accept/reject and dt per trajectory (solveras.py:384-410,
+//         457-554; evolve 605-650)
+// begin/ctrl/snap keep each trajectory's own controller state, so finished
+// or rejected trajectories drop out of (or repeat) work per launch while the
+// others advance.  The host captures the sequence once into a hipGraph and
+// replays it until no trajectory is running.
+#pragma once
+#include "rks_kernels.h"
+
+#define NLK_AC2D 3
+
+// per-trajectory flags of the current attempt
+struct rks2_state {
+  int active;     // takes part in this attempt
+  int need_nl1;   // NL1 = nl(u) first (start, or RECOMPUTE after an accept)
+  int need_coef;  // h changed: regenerate coefficient arrays
+  int store;      // copy the current state to snapshot snap_idx
+  int snap_idx;
+  int pad;
+};
+
+struct rks2_launch {
+  const rks_tableau* tab;
+  const rks_ctrl* ctl;
+  rks_inst* inst;
+  const rks_io* io;
+  rks2_state* st;
+  const void* tw;    // exp(-2 pi i x / N), x < N, state precision
+  double* pmax;      // [ninst][ncg] partial max |u_new|
+  double* psum;      // [ninst][nnb][2] partial masked sums
+  int* active_count;
+  size_t off_cand0, off_cand1, off_coef, off_T;  // byte offsets in an instance workspace
+  int ninst, mode, ncg, nnb;
+  int lowp;  // NL arrays stored in FP32
+};
+
+template <typename R, int LOG2>
+struct Geo2 {
+  static constexpr int N = 1 << LOG2;
+  static constexpr int NXH = N / 2 + 1;
+  static constexpr int M = N * NXH;
+  static constexpr int PS = rksfft::padded_size(N);
+  static constexpr int TWS = rksfft::TwSplit<LOG2>::SIZE;
+  // column kernel: CG columns of N points, CTG threads per column
+  static constexpr int CG = sizeof(R) == 4 ? 16 : 8;
+  static constexpr int CTG = (N / 2 < 512 / CG) ? N / 2 : 512 / CG;
+  static constexpr int CT = CG * CTG;
+  static constexpr int CEPT = N / CTG;  // tile elements per thread
+  static constexpr int CCH = CEPT < 8 ? CEPT : 8;
+  static constexpr int NCG = (NXH + CG - 1) / CG;
+  static constexpr int COL_LDS = (CG * PS + TWS) * (int)sizeof(cx<R>);
+  // row kernel: RG complex transforms (2 RG rows), RTG threads each
+  static constexpr int RG = sizeof(R) == 4 ? 8 : 4;
+  static constexpr int RTG = (N / 2 < 512 / RG) ? N / 2 : 512 / RG;
+  static constexpr int RT = RG * RTG;
+  static constexpr int NRG = N / (2 * RG);
+  static constexpr int ROW_LDS = (RG * PS + TWS) * (int)sizeof(cx<R>);
+  // norm kernel
+  static constexpr int NT = 256, NE = 8;
+  static constexpr int NNB = (M + NT * NE - 1) / (NT * NE);
+  static_assert(N >= 2 * RG, "grid too small for the row kernel");
+  static_assert(CEPT >= 2, "column transform needs two elements per thread");
+};
+
+__device__ __forceinline__ unsigned rks2_smap(const rks_inst* in) {
+  unsigned smap = 0;
+#pragma unroll
+  for (int s = 0; s < RKS_MAX_SLOTS; ++s) smap |= (unsigned)(in->slot_map[s] & 15) << (4 * s);
+  return smap;
+}
+
+// current state and candidate buffer of an instance
+template <typename R>
+__device__ __forceinline__ void rks2_states(const rks2_launch& P, const rks_inst* in, const rks_io& io,
+                                            const cx<R>** u, cx<R>** cand) {
+  char* ws = (char*)in->ws;
+  cx<R>* c0 = (cx<R>*)(ws + P.off_cand0);
+  cx<R>* c1 = (cx<R>*)(ws + P.off_cand1);
+  if (P.mode == RKS_MODE_EVOLVE) {
+    const int ub = in->ubuf;
+    *u = ub < 0 ? (const cx<R>*)io.u_in : (ub == 0 ? c0 : c1);
+    *cand = ub == 0 ? c1 : c0;
+  } else {
+    *u = (const cx<R>*)io.u_in;
+    *cand = (cx<R>*)io.u_out;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// col: [forward column FFTs of T -> NL slot fwd_slot] then [argument of stage
+// `arg` (arg == -1: u itself; arg == -2: none) -> candidate (last stage) and/or
+// inverse column FFTs -> T].  fwd_need_nl1: the forward part only for
+// trajectories that evaluated NL1 in this attempt; prelude: the whole kernel
+// only for those.
+// ---------------------------------------------------------------------------
+template <typename R, typename RN, int LOG2>
+__global__ __launch_bounds__((Geo2<R, LOG2>::CT)) void k2_col(rks2_launch P, int fwd_slot, int fwd_need_nl1,
+                                                            int arg, int prelude) {
+  using G = Geo2<R, LOG2>;
+  using C = cx<R>;
+  using CN = cx<RN>;
+  using rksfft::pad;
+  constexpr int N = G::N, NXH = G::NXH, M = G::M, CG = G::CG, CT = G::CT, CEPT = G::CEPT;
+  constexpr int CH = G::CCH, NCH = CEPT / CH, RS = CT / CG, PS = G::PS;
+  extern __shared__ __align__(16) unsigned char rks2_smem[];
+  __shared__ double red[2 * (CT / 64 + 1)];
+  const int inst = blockIdx.y;
+  const rks2_state st = P.st[inst];
+  if (!st.active || (prelude && !st.need_nl1)) return;
+  C* buf = (C*)rks2_smem;
+  C* twl = buf + CG * PS;
+  const rks_tableau& tb = *P.tab;
+  rks_inst* in = P.inst + inst;
+  const rks_io& io = P.io[inst];
+  char* ws = (char*)in->ws;
+  const double h = in->h;
+  const unsigned smap = rks2_smap(in);
+  CN* const slot_base = (CN*)ws;
+  auto slot = [&](int s) -> CN* { return slot_base + (size_t)((smap >> (4 * s)) & 15u) * M; };
+  const C* u;
+  C* cand;
+  rks2_states<R>(P, in, io, &u, &cand);
+  C* Tb = (C*)(ws + P.off_T);
+  const char* cbase = ws + P.off_coef;
+  rksfft::load_twiddles<R, LOG2, CT>(twl, (const C*)P.tw);
+  const int t = threadIdx.x, c = t % CG, r0 = t / CG;
+  const int col = blockIdx.x * CG + c;
+  const bool valid = col < NXH;
+  C* const bc = buf + c * PS;
+  const C zero = mkc((R)0, (R)0);
+  __syncthreads();
+  if (fwd_slot >= 0 && (!fwd_need_nl1 || st.need_nl1)) {
+#pragma unroll
+    for (int i = 0; i < CEPT; ++i) {
+      const int r = r0 + i * RS;
+      bc[pad(r)] = valid ? Tb[r * NXH + col] : zero;
+    }
+    __syncthreads();
+    rksfft::fft_multi<R, N, G::CTG, CG, false, LOG2, RKS_MAXR>(buf, twl);
+    if (valid) {
+      if (P.mode == RKS_MODE_NL) {
+        C* dst = (C*)io.u_out;
+#pragma unroll
+        for (int i = 0; i < CEPT; ++i) {
+          const int r = r0 + i * RS;
+          dst[r * NXH + col] = bc[pad(r)];
+        }
+      } else {
+        CN* dst = slot(fwd_slot);
+#pragma unroll
+        for (int i = 0; i < CEPT; ++i) {
+          const int r = r0 + i * RS;
+          dst[r * NXH + col] = cvt<RN>(bc[pad(r)]);
+        }
+      }
+    }
+    __syncthreads();
+  }
+  if (arg < -1) return;
+  const bool has_nl = arg < 0 || tb.st[arg].has_nl;
+  const bool last = arg >= 0 && arg == tb.nstage - 1;
+  double mx = 0.0;
+#pragma unroll 1
+  for (int ch = 0; ch < NCH; ++ch) {
+    const int rr = r0 + ch * CH * RS;
+    const int m0 = rr * NXH + col;
+    C acc[CH];
+    if (arg < 0) {
+#pragma unroll
+      for (int i = 0; i < CH; ++i) acc[i] = valid ? u[m0 + i * RS * NXH] : zero;
+    } else {
+      stage_chunk<R, RN, CH>(tb, tb.st[arg], u, slot, cbase, (size_t)M, m0, RS * NXH, valid ? M : 0, h, acc);
+    }
+#pragma unroll
+    for (int i = 0; i < CH; ++i) {
+      if (last && valid) {
+        cand[m0 + i * RS * NXH] = acc[i];
+        mx = nanmax(mx, hypot((double)acc[i].re, (double)acc[i].im));
+      }
+      if (has_nl) bc[pad(rr + i * RS)] = valid ? acc[i] : zero;
+    }
+  }
+  if (last && tb.adaptive) {
+    mx = wg_max<CT>(mx, red);
+    if (t == 0) P.pmax[inst * P.ncg + blockIdx.x] = mx;
+  }
+  if (has_nl) {
+    __syncthreads();
+    rksfft::fft_multi<R, N, G::CTG, CG, true, LOG2, RKS_MAXR>(buf, twl);
+    if (valid) {
+#pragma unroll
+      for (int i = 0; i < CEPT; ++i) {
+        const int r = r0 + i * RS;
+        Tb[r * NXH + col] = bc[pad(r)];
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// row: irfft along rows, w - c w^3, rfft along rows; two rows per complex
+// transform (z = w1 + i w2).  T holds the column-inverse-transformed spectrum
+// and is overwritten in place with the rows' forward spectra.
+// ---------------------------------------------------------------------------
+template <typename R, int LOG2>
+__global__ __launch_bounds__((Geo2<R, LOG2>::RT)) void k2_row(rks2_launch P, int prelude) {
+  using G = Geo2<R, LOG2>;
+  using C = cx<R>;
+  using rksfft::pad;
+  constexpr int N = G::N, NXH = G::NXH, RG = G::RG, RT = G::RT, PS = G::PS;
+  extern __shared__ __align__(16) unsigned char rks2_smem[];
+  const int inst = blockIdx.y;
+  const rks2_state st = P.st[inst];
+  if (!st.active || (prelude && !st.need_nl1)) return;
+  C* buf = (C*)rks2_smem;
+  C* twl = buf + RG * PS;
+  const rks_inst* in = P.inst + inst;
+  C* Tb = (C*)((char*)in->ws + P.off_T);
+  const R c3 = (R)in->nl_coef;
+  const R inv = (R)(1.0 / ((double)N * (double)N));  // irfft2 normalisation (exact)
+  rksfft::load_twiddles<R, LOG2, RT>(twl, (const C*)P.tw);
+  const int t = threadIdx.x;
+  const int row0 = blockIdx.x * 2 * RG;
+  for (int e = t; e < RG * NXH; e += RT) {
+    const int g = e / NXH, k = e - g * NXH;
+    const C a = Tb[(row0 + 2 * g) * NXH + k];
+    const C b = Tb[(row0 + 2 * g + 1) * NXH + k];
+    C* bg = buf + g * PS;
+    // full spectrum of w1 + i w2 from the two half spectra; DC and Nyquist keep
+    // only their real parts, as irfft does
+    if (k == 0 || k == N / 2) {
+      bg[pad(k)] = mkc(a.re, b.re);
+    } else {
+      bg[pad(k)] = mkc(a.re - b.im, a.im + b.re);
+      bg[pad(N - k)] = mkc(a.re + b.im, b.re - a.im);
+    }
+  }
+  __syncthreads();
+  rksfft::fft_multi<R, N, G::RTG, RG, true, LOG2, RKS_MAXR>(buf, twl);
+  for (int e = t; e < RG * N; e += RT) {
+    const int g = e / N, j = e - g * N;
+    C* bg = buf + g * PS;
+    const C z = bg[pad(j)];
+    const R w1 = z.re * inv, w2 = z.im * inv;
+    bg[pad(j)] = mkc(w1 - c3 * (w1 * w1 * w1), w2 - c3 * (w2 * w2 * w2));
+  }
+  __syncthreads();
+  rksfft::fft_multi<R, N, G::RTG, RG, false, LOG2, RKS_MAXR>(buf, twl);
+  for (int e = t; e < RG * NXH; e += RT) {
+    const int g = e / NXH, k = e - g * NXH;
+    const C* bg = buf + g * PS;
+    const C z = bg[pad(k)];
+    const C y = bg[pad((N - k) & (N - 1))];
+    // Z = F1 + i F2 with F1, F2 Hermitian: F1 = (Z + conj Y)/2, F2 = (Z - conj Y)/(2i)
+    const R hf = (R)0.5;
+    Tb[(row0 + 2 * g) * NXH + k] = mkc((z.re + y.re) * hf, (z.im - y.im) * hf);
+    Tb[(row0 + 2 * g + 1) * NXH + k] = mkc((z.im + y.im) * hf, (y.re - z.re) * hf);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// coefficients (FP64 arithmetic) of trajectories whose h changed
+// ---------------------------------------------------------------------------
+template <typename R, typename RN>
+__global__ __launch_bounds__(256) void k2_coef(rks2_launch P, int M) {
+  const int inst = blockIdx.y;
+  const rks2_state st = P.st[inst];
+  if (!st.active || !st.need_coef) return;
+  const rks_inst* in = P.inst + inst;
+  char* cbase = (char*)in->ws + P.off_coef;
+  for (int m = blockIdx.x * 256 + threadIdx.x; m < M; m += gridDim.x * 256)
+    coef_mode<R, RN>(*P.tab, *P.ctl, in->L, in->L_complex, m, M, in->h, cbase);
+}
+
+// ---------------------------------------------------------------------------
+// norm: masked sums |u_new|^2, |err|^2 over modes with |u_new|/max > cutoff
+// ---------------------------------------------------------------------------
+template <typename R, typename RN, int LOG2>
+__global__ __launch_bounds__(256) void k2_norm(rks2_launch P) {
+  using G = Geo2<R, LOG2>;
+  using C = cx<R>;
+  using CN = cx<RN>;
+  constexpr int M = G::M, NT = G::NT, NE = G::NE;
+  __shared__ double red[2 * (NT / 64 + 1)];
+  const int inst = blockIdx.y;
+  const rks2_state st = P.st[inst];
+  const rks_tableau& tb = *P.tab;
+  if (!st.active || !tb.adaptive) return;
+  const rks_inst* in = P.inst + inst;
+  const rks_io& io = P.io[inst];
+  char* ws = (char*)in->ws;
+  const unsigned smap = rks2_smap(in);
+  CN* const slot_base = (CN*)ws;
+  auto slot = [&](int s) -> CN* { return slot_base + (size_t)((smap >> (4 * s)) & 15u) * M; };
+  const C* u;
+  C* cand;
+  rks2_states<R>(P, in, io, &u, &cand);
+  const double* pm = P.pmax + (size_t)inst * P.ncg;
+  double mx = pm[0];
+  for (int j = 1; j < P.ncg; ++j) mx = nanmax(mx, pm[j]);
+  const int t = threadIdx.x;
+  const int m0 = blockIdx.x * NT * NE + t;
+  C ev[NE];
+  error_chunk<R, RN, NE>(tb, slot, ws + P.off_coef, (size_t)M, m0, NT, M, in->h, ev);
+  const double cut = P.ctl->adapt_cutoff;
+  double su = 0.0, se = 0.0;
+#pragma unroll
+  for (int i = 0; i < NE; ++i) {
+    const int m = m0 + i * NT;
+    if (m < M) {
+      const C kv = cand[m];
+      const double kr = (double)kv.re, ki = (double)kv.im;
+      if (hypot(kr, ki) / mx > cut) {
+        const double er = (double)ev[i].re, ei = (double)ev[i].im;
+        su += kr * kr + ki * ki;
+        se += er * er + ei * ei;
+      }
+    }
+  }
+  wg_sum2<NT>(su, se, red);
+  if (t == 0) {
+    double* ps = P.psum + 2 * ((size_t)inst * P.nnb + blockIdx.x);
+    ps[0] = su;
+    ps[1] = se;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// per-trajectory control (one block; a thread per trajectory)
+// ---------------------------------------------------------------------------
+// set-up of a launch: evolve / resume / step (run_instance's set-up)
+__global__ void k2_init(rks2_launch P) {
+  const rks_tableau& tb = *P.tab;
+  for (int i = threadIdx.x; i < P.ninst; i += blockDim.x) {
+    const rks_io io = P.io[i];
+    rks_inst& in = P.inst[i];
+    rks2_state& st = P.st[i];
+    st.active = st.need_nl1 = st.need_coef = st.store = st.snap_idx = 0;
+    if (io.reset) {
+      in.started = 0;
+      in.accept = 0;
+      in.h_coeff = __builtin_nan("");
+      for (int s = 0; s < RKS_MAX_SLOTS; ++s) in.slot_map[s] = s;
+    }
+    if (P.mode == RKS_MODE_EVOLVE && !io.resume) {
+      in.tc = io.t0;
+      in.tf = io.tf;
+      in.h = io.h_in;
+      in.steps = in.attempts = in.rejects = 0;
+      in.log_count = in.snap_count = 0;
+      in.nl_calls = in.coef_updates = 0;
+      in.ubuf = -1;
+      in.status = RKS_ST_RUNNING;
+      if (tb.adaptive && in.tc + in.h > in.tf) in.h = in.tf - in.tc;  // solveras.py:620-621
+      if (in.store_freq > 0) {
+        st.store = 1;
+        st.snap_idx = 0;
+        in.snap_t[0] = in.tc;
+        in.snap_count = 1;
+      }
+    } else if (P.mode == RKS_MODE_EVOLVE) {
+      in.status = RKS_ST_RUNNING;
+    } else if (P.mode == RKS_MODE_STEP) {
+      in.h = io.h_in;
+      in.status = RKS_ST_RUNNING;
+      in.attempts = in.rejects = 0;
+    } else {  // RKS_MODE_NL
+      st.active = 1;
+      st.need_nl1 = 1;
+    }
+    in.numloops = 0;
+  }
+}
+
+// start of an attempt: who runs, coefficient and NL1 needs, FSAL slot swap
+__global__ void k2_begin(rks2_launch P) {
+  __shared__ int cnt[4];
+  const rks_tableau& tb = *P.tab;
+  int n_active = 0;
+  for (int i = threadIdx.x; i < P.ninst; i += blockDim.x) {
+    rks2_state& st = P.st[i];
+    rks_inst& in = P.inst[i];
+    st.store = 0;
+    int active = 0;
+    if (in.status == RKS_ST_RUNNING) {
+      const bool evolve = P.mode == RKS_MODE_EVOLVE;
+      if (evolve && !(in.tc < in.tf)) in.status = RKS_ST_OK;
+      else if (evolve && in.store_freq > 0 && in.snap_count >= in.snap_cap) in.status = RKS_ST_PAUSED;
+      else active = 1;
+    }
+    st.active = active;
+    st.need_nl1 = 0;
+    st.need_coef = 0;
+    if (!active) continue;
+    ++n_active;
+    if (in.h != in.h_coeff) {
+      st.need_coef = 1;
+      in.h_coeff = in.h;
+      ++in.coef_updates;
+    }
+    if (!in.started) {
+      st.need_nl1 = 1;
+      in.started = 1;
+    } else if (in.accept || !tb.adaptive) {
+      if (tb.nl1_policy == RKS_NL1_RECOMPUTE) {
+        st.need_nl1 = 1;
+      } else {
+        const int f = tb.fsal_slot, a = in.slot_map[0];
+        in.slot_map[0] = in.slot_map[f];
+        in.slot_map[f] = a;
+      }
+    }
+    int nl = st.need_nl1;
+    for (int s = 0; s < tb.nstage; ++s) nl += tb.st[s].has_nl ? 1 : 0;
+    in.nl_calls += nl;
+  }
+  // block sum of n_active (blockDim.x = 256)
+  for (int o = 32; o >= 1; o >>= 1) n_active += __shfl_xor(n_active, o, 64);
+  if ((threadIdx.x & 63) == 0) cnt[threadIdx.x >> 6] = n_active;
+  __syncthreads();
+  if (threadIdx.x == 0) *P.active_count = cnt[0] + cnt[1] + cnt[2] + cnt[3];
+}
+
+// end of an attempt: error ratio, accept/reject, dt, time, snapshots
+__global__ void k2_ctrl(rks2_launch P) {
+  const rks_tableau& tb = *P.tab;
+  const rks_ctrl& ct = *P.ctl;
+  for (int i = threadIdx.x; i < P.ninst; i += blockDim.x) {
+    rks2_state& st = P.st[i];
+    if (!st.active) continue;
+    rks_inst& in = P.inst[i];
+    const double h = in.h;
+    double h_suggest = h;
+    if (tb.adaptive) {
+      double su = 0.0, se = 0.0;
+      const double* ps = P.psum + 2 * (size_t)i * P.nnb;
+      for (int b = 0; b < P.nnb; ++b) {
+        su += ps[2 * b];
+        se += ps[2 * b + 1];
+      }
+      const double sv = rks_error_ratio(su, se, ct.epsilon, ct.safety_f, tb.q, P.lowp != 0);
+      ++in.attempts;
+      if (in.log_count < in.log_cap) {
+        in.log[2 * in.log_count] = h;
+        in.log[2 * in.log_count + 1] = sv;
+      }
+      ++in.log_count;
+      if (isinf(sv) || isnan(sv) || sv < 1.0) {  // reject (solveras.py:490-506)
+        in.accept = 0;
+        ++in.rejects;
+        double hn;
+        if (isinf(sv) || isnan(sv)) {
+          hn = ct.min_s * h;
+        } else {
+          double sc = sv > ct.min_s ? sv : ct.min_s;
+          sc = sc < ct.decr_f ? sc : ct.decr_f;
+          hn = sc * h;
+        }
+        in.h = hn;
+        ++in.numloops;
+        if (in.numloops > ct.max_loops) in.status = RKS_ST_MAXLOOPS;
+        else if (hn < ct.minh) in.status = RKS_ST_MINSTEP;
+        continue;
+      }
+      in.accept = 1;  // accept (solveras.py:543-554)
+      const double sc = sv < ct.max_s ? sv : ct.max_s;
+      h_suggest = (sc > ct.incr_f) ? sc * h : h;
+    } else {
+      in.accept = 1;
+    }
+    in.numloops = 0;
+    in.h_used = h;
+    in.h_next = h_suggest;
+    if (P.mode != RKS_MODE_EVOLVE) {
+      in.status = RKS_ST_OK;
+      ++in.steps;
+      continue;
+    }
+    in.ubuf = in.ubuf == 0 ? 1 : 0;
+    const double tprev = in.tc;
+    in.tc += h;
+    ++in.steps;
+    if (!(in.tc > tprev)) {  // h below the resolution of tc
+      in.status = RKS_ST_MINSTEP;
+      continue;
+    }
+    if (tb.adaptive) in.h = (in.tc + h_suggest > in.tf) ? in.tf - in.tc : h_suggest;  // solveras.py:638-641
+    if (in.store_freq > 0 && (in.steps % in.store_freq) == 0) {
+      st.store = 1;
+      st.snap_idx = in.snap_count;
+      in.snap_t[in.snap_count] = in.tc;
+      ++in.snap_count;
+    }
+  }
+}
+
+// snapshot copy of the current state (after ctrl / init)
+template <typename R>
+__global__ __launch_bounds__(256) void k2_snap(rks2_launch P, int M) {
+  const int inst = blockIdx.y;
+  const rks2_state st = P.st[inst];
+  if (!st.store) return;
+  const rks_inst* in = P.inst + inst;
+  const rks_io& io = P.io[inst];
+  const cx<R>* u;
+  cx<R>* cand;
+  rks2_states<R>(P, in, io, &u, &cand);
+  cx<R>* dst = (cx<R>*)in->snap + (size_t)st.snap_idx * M;
+  for (int m = blockIdx.x * 256 + threadIdx.x; m < M; m += gridDim.x * 256) dst[m] = u[m];
+}
+
+// final state of an evolve (not for paused trajectories)
+template <typename R>
+__global__ __launch_bounds__(256) void k2_out(rks2_launch P, int M) {
+  const int inst = blockIdx.y;
+  const rks_inst* in = P.inst + inst;
+  if (in->status == RKS_ST_PAUSED) return;
+  const rks_io& io = P.io[inst];
+  const cx<R>* u;
+  cx<R>* cand;
+  rks2_states<R>(P, in, io, &u, &cand);
+  cx<R>* dst = (cx<R>*)io.u_out;
+  for (int m = blockIdx.x * 256 + threadIdx.x; m < M; m += gridDim.x * 256) dst[m] = u[m];
+}

@@ -1,0 +1,236 @@
+// Launch sequences of the 2-D rfft2 engine and their hipGraph replay.
+#include <cstdlib>
+#include <cstring>
+
+#include "rkstiff_amd.h"
+#include "rks_2d.h"
+#include "rks_2d_host.h"
+
+struct rks2_ctx {
+  int prec = 0, log2n = 0, ninst = 0, M = 0;
+  rks2_launch L;
+  rks2_state* d_st = nullptr;
+  double* d_pmax = nullptr;
+  double* d_psum = nullptr;
+  int* d_active = nullptr;
+  int* h_active = nullptr;  // pinned
+  hipStream_t cap = nullptr;
+  hipGraph_t graph[2] = {nullptr, nullptr};  // evolve, step
+  hipGraphExec_t exec[2] = {nullptr, nullptr};
+  bool use_graph = true;
+};
+
+template <typename R, typename RN, int LOG2>
+struct Eng2 {
+  using G = Geo2<R, LOG2>;
+  static hipError_t setup() {
+    hipError_t e = hipFuncSetAttribute((const void*)k2_col<R, RN, LOG2>,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, G::COL_LDS);
+    if (e != hipSuccess) return e;
+    return hipFuncSetAttribute((const void*)k2_row<R, LOG2>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               G::ROW_LDS);
+  }
+  static int ncg() { return G::NCG; }
+  static int nnb() { return G::NNB; }
+  static void col(const rks2_launch& L, hipStream_t s, int fwd, int cond, int arg, int prelude) {
+    hipLaunchKernelGGL((k2_col<R, RN, LOG2>), dim3(G::NCG, L.ninst), dim3(G::CT), G::COL_LDS, s, L, fwd, cond, arg,
+                       prelude);
+  }
+  static void row(const rks2_launch& L, hipStream_t s, int prelude) {
+    hipLaunchKernelGGL((k2_row<R, LOG2>), dim3(G::NRG, L.ninst), dim3(G::RT), G::ROW_LDS, s, L, prelude);
+  }
+  static int copy_blocks() { return (G::M + 256 * 16 - 1) / (256 * 16); }
+  static void snap(const rks2_launch& L, hipStream_t s) {
+    hipLaunchKernelGGL((k2_snap<R>), dim3(copy_blocks(), L.ninst), dim3(256), 0, s, L, G::M);
+  }
+  static void out(const rks2_launch& L, hipStream_t s) {
+    hipLaunchKernelGGL((k2_out<R>), dim3(copy_blocks(), L.ninst), dim3(256), 0, s, L, G::M);
+  }
+  // one attempt of every running trajectory
+  static void attempt(const rks2_launch& L, const rks_tableau& tb, hipStream_t s) {
+    hipLaunchKernelGGL(k2_begin, dim3(1), dim3(256), 0, s, L);
+    hipLaunchKernelGGL((k2_coef<R, RN>), dim3((G::M + 256 * 8 - 1) / (256 * 8), L.ninst), dim3(256), 0, s, L,
+                       G::M);
+    // NL1 = nl(u) where needed (start; after an accept for RECOMPUTE methods)
+    col(L, s, -1, 0, -1, 1);
+    row(L, s, 1);
+    for (int k = 0; k < tb.nstage; ++k) {
+      int fwd = -1, cond = 0;
+      if (k == 0) {
+        fwd = 0;
+        cond = 1;
+      } else if (tb.st[k - 1].has_nl) {
+        fwd = tb.st[k - 1].nl_slot;
+      }
+      col(L, s, fwd, cond, k, 0);
+      if (tb.st[k].has_nl) row(L, s, 0);
+    }
+    if (tb.st[tb.nstage - 1].has_nl) col(L, s, tb.st[tb.nstage - 1].nl_slot, 0, -2, 0);
+    if (tb.adaptive) hipLaunchKernelGGL((k2_norm<R, RN, LOG2>), dim3(G::NNB, L.ninst), dim3(G::NT), 0, s, L);
+    hipLaunchKernelGGL(k2_ctrl, dim3(1), dim3(256), 0, s, L);
+    snap(L, s);
+  }
+  static void nl(const rks2_launch& L, hipStream_t s) {
+    col(L, s, -1, 0, -1, 1);
+    row(L, s, 1);
+    col(L, s, 0, 1, -2, 0);
+  }
+};
+
+template <typename R, typename RN, typename F>
+static bool with_size(int log2n, F&& f) {
+  switch (log2n) {
+    case 5: f(Eng2<R, RN, 5>{}); return true;
+    case 6: f(Eng2<R, RN, 6>{}); return true;
+    case 7: f(Eng2<R, RN, 7>{}); return true;
+    case 8: f(Eng2<R, RN, 8>{}); return true;
+    case 9: f(Eng2<R, RN, 9>{}); return true;
+    default: return false;
+  }
+}
+
+template <typename F>
+static bool with_engine(int prec, int log2n, F&& f) {
+  if (prec == RKS_F64) return with_size<double, double>(log2n, f);
+  if (prec == RKS_F32) return with_size<float, float>(log2n, f);
+  if (prec == RKS_MIXED) return with_size<double, float>(log2n, f);
+  return false;
+}
+
+int rks2_supported(int prec, int log2n) {
+  return with_engine(prec, log2n, [](auto) {}) ? 1 : 0;
+}
+
+int rks2_create(rks2_ctx** out, int prec, int log2n, int ninst, const rks_tableau* d_tab, const rks_ctrl* d_ctl,
+                rks_inst* d_inst, const rks_io* d_io, const void* d_tw, size_t off_cand0, size_t off_cand1,
+                size_t off_coef, size_t off_T) {
+  *out = nullptr;
+  rks2_ctx* c = new rks2_ctx();
+  c->prec = prec;
+  c->log2n = log2n;
+  c->ninst = ninst;
+  c->M = rks2_modes(log2n);
+  int ncg = 0, nnb = 0;
+  hipError_t e = hipSuccess;
+  if (!with_engine(prec, log2n, [&](auto eng) {
+        using E = decltype(eng);
+        ncg = E::ncg();
+        nnb = E::nnb();
+        e = E::setup();
+      })) {
+    delete c;
+    return 1;
+  }
+  if (e == hipSuccess) e = hipMalloc(&c->d_st, sizeof(rks2_state) * ninst);
+  if (e == hipSuccess) e = hipMalloc(&c->d_pmax, sizeof(double) * ninst * ncg);
+  if (e == hipSuccess) e = hipMalloc(&c->d_psum, sizeof(double) * 2 * ninst * nnb);
+  if (e == hipSuccess) e = hipMalloc(&c->d_active, sizeof(int));
+  if (e == hipSuccess) e = hipHostMalloc((void**)&c->h_active, sizeof(int), hipHostMallocDefault);
+  if (e == hipSuccess) e = hipMemset(c->d_st, 0, sizeof(rks2_state) * ninst);
+  if (e != hipSuccess) {
+    rks2_destroy(c);
+    return 2;
+  }
+  rks2_launch& L = c->L;
+  std::memset(&L, 0, sizeof(L));
+  L.tab = d_tab;
+  L.ctl = d_ctl;
+  L.inst = d_inst;
+  L.io = d_io;
+  L.st = c->d_st;
+  L.tw = d_tw;
+  L.pmax = c->d_pmax;
+  L.psum = c->d_psum;
+  L.active_count = c->d_active;
+  L.off_cand0 = off_cand0;
+  L.off_cand1 = off_cand1;
+  L.off_coef = off_coef;
+  L.off_T = off_T;
+  L.ninst = ninst;
+  L.ncg = ncg;
+  L.nnb = nnb;
+  L.lowp = prec != RKS_F64;
+  const char* ng = std::getenv("RKS_NO_GRAPH");
+  c->use_graph = !(ng && ng[0] == '1');
+  *out = c;
+  return 0;
+}
+
+// the attempt sequence of `mode`, captured once
+static hipGraphExec_t attempt_graph(rks2_ctx* c, const rks_tableau& tb, int mode) {
+  const int gi = mode == RKS_MODE_EVOLVE ? 0 : 1;
+  if (c->exec[gi] || !c->use_graph) return c->exec[gi];
+  if (!c->cap && hipStreamCreateWithFlags(&c->cap, hipStreamNonBlocking) != hipSuccess) {
+    c->use_graph = false;
+    return nullptr;
+  }
+  rks2_launch L = c->L;
+  L.mode = mode;
+  if (hipStreamBeginCapture(c->cap, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+    c->use_graph = false;
+    return nullptr;
+  }
+  with_engine(c->prec, c->log2n, [&](auto eng) { decltype(eng)::attempt(L, tb, c->cap); });
+  hipGraph_t g = nullptr;
+  if (hipStreamEndCapture(c->cap, &g) != hipSuccess || !g) {
+    c->use_graph = false;
+    return nullptr;
+  }
+  hipGraphExec_t x = nullptr;
+  if (hipGraphInstantiate(&x, g, nullptr, nullptr, 0) != hipSuccess) {
+    (void)hipGraphDestroy(g);
+    c->use_graph = false;
+    return nullptr;
+  }
+  c->graph[gi] = g;
+  c->exec[gi] = x;
+  return x;
+}
+
+hipError_t rks2_run(rks2_ctx* c, const rks_tableau& tb, int mode, hipStream_t s) {
+  rks2_launch L = c->L;
+  L.mode = mode;
+  hipLaunchKernelGGL(k2_init, dim3(1), dim3(256), 0, s, L);
+  if (mode == RKS_MODE_NL) {
+    with_engine(c->prec, c->log2n, [&](auto eng) { decltype(eng)::nl(L, s); });
+    return hipGetLastError();
+  }
+  if (mode == RKS_MODE_EVOLVE) with_engine(c->prec, c->log2n, [&](auto eng) { decltype(eng)::snap(L, s); });
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipGraphExec_t g = attempt_graph(c, tb, mode);
+  // every attempt either advances tc (bounded by tf and the resolution guard)
+  // or counts towards max_loops, so the loop ends; the cap only guards bugs
+  const long long cap = 200000000LL;
+  const int K = 4;  // attempts between completion checks
+  for (long long it = 0;; it += K) {
+    for (int k = 0; k < K; ++k) {
+      if (g) e = hipGraphLaunch(g, s);
+      else with_engine(c->prec, c->log2n, [&](auto eng) { decltype(eng)::attempt(L, tb, s); });
+      if (e == hipSuccess) e = hipGetLastError();
+      if (e != hipSuccess) return e;
+    }
+    e = hipMemcpyAsync(c->h_active, c->d_active, sizeof(int), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return e;
+    if (*c->h_active == 0) break;
+    if (it > cap) return hipErrorUnknown;
+  }
+  if (mode == RKS_MODE_EVOLVE) with_engine(c->prec, c->log2n, [&](auto eng) { decltype(eng)::out(L, s); });
+  return hipGetLastError();
+}
+
+void rks2_destroy(rks2_ctx* c) {
+  if (!c) return;
+  for (int i = 0; i < 2; ++i) {
+    if (c->exec[i]) (void)hipGraphExecDestroy(c->exec[i]);
+    if (c->graph[i]) (void)hipGraphDestroy(c->graph[i]);
+  }
+  if (c->cap) (void)hipStreamDestroy(c->cap);
+  if (c->d_st) (void)hipFree(c->d_st);
+  if (c->d_pmax) (void)hipFree(c->d_pmax);
+  if (c->d_psum) (void)hipFree(c->d_psum);
+  if (c->d_active) (void)hipFree(c->d_active);
+  if (c->h_active) (void)hipHostFree(c->h_active);
+  delete c;
+}

@@ -1,0 +1,27 @@
+// Host side of the 2-D rfft2 engine (kernels in rks_2d.h, launches in rks_2d.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+
+#include "rks_types.h"
+
+struct rks2_launch;
+struct rks2_ctx;
+
+// 1 if kernels exist for this precision and N = 2^log2n
+int rks2_supported(int prec, int log2n);
+// modes of an N x N rfft2 grid
+inline int rks2_modes(int log2n) { return (1 << log2n) * ((1 << log2n) / 2 + 1); }
+
+// Allocates the per-trajectory flags and reduction partials and sets kernel
+// attributes.  The workspace of instance i holds (byte offsets)
+//   NL slots at 0 | cand0 at off_cand0 | cand1 at off_cand1 | coefficients at
+//   off_coef | transform buffer T (M state-precision complex) at off_T.
+int rks2_create(rks2_ctx** out, int prec, int log2n, int ninst, const rks_tableau* d_tab, const rks_ctrl* d_ctl,
+                rks_inst* d_inst, const rks_io* d_io, const void* d_tw, size_t off_cand0, size_t off_cand1,
+                size_t off_coef, size_t off_T);
+// One launch set: mode RKS_MODE_EVOLVE / RKS_MODE_STEP (attempts until every
+// trajectory has finished) or RKS_MODE_NL.  Synchronises `stream`.
+hipError_t rks2_run(rks2_ctx* c, const rks_tableau& tab, int mode, hipStream_t stream);
+void rks2_destroy(rks2_ctx* c);

@@ -130,11 +130,11 @@ struct pick_radix {
 
 // One Stockham pass.  Twiddle W_{NS*RAD}^{r k} = W_N^{r k (P/(NS RAD)) TSCALE}, where
 // TSCALE = N / P maps a P-point transform onto the N-point table.
+// t: the thread's index among the T threads working on this transform.
 template <typename R, int P, int T, int NS, int RAD, bool INV, int LOG2N, int TSCALE>
-__device__ __forceinline__ void pass(cx<R>* buf, const cx<R>* twl) {
+__device__ __forceinline__ void pass(cx<R>* buf, const cx<R>* twl, int t) {
   constexpr int NB = P / (RAD * T);
   static_assert(NB >= 1, "too few elements per thread");
-  const int t = threadIdx.x;
   cx<R> v[NB][RAD];
 #pragma unroll
   for (int b = 0; b < NB; ++b) {
@@ -161,14 +161,14 @@ __device__ __forceinline__ void pass(cx<R>* buf, const cx<R>* twl) {
 }
 
 template <typename R, int P, int T, int NS, bool INV, int LOG2N, int TSCALE, int MAXR>
-__device__ __forceinline__ void chain(cx<R>* buf, const cx<R>* twl) {
+__device__ __forceinline__ void chain(cx<R>* buf, const cx<R>* twl, int t) {
   if constexpr (NS < P) {
     constexpr int E = P / T;
     constexpr int R0 = pick_radix<E, MAXR>::v;
     constexpr int REM = P / NS;
     constexpr int RAD = (R0 <= REM) ? R0 : REM;
-    pass<R, P, T, NS, RAD, INV, LOG2N, TSCALE>(buf, twl);
-    chain<R, P, T, NS * RAD, INV, LOG2N, TSCALE, MAXR>(buf, twl);
+    pass<R, P, T, NS, RAD, INV, LOG2N, TSCALE>(buf, twl, t);
+    chain<R, P, T, NS * RAD, INV, LOG2N, TSCALE, MAXR>(buf, twl, t);
   }
 }
 
@@ -178,7 +178,18 @@ template <typename R, int P, int T, bool INV, int LOG2N, int MAXR = 32>
 __device__ __forceinline__ void fft(cx<R>* buf, const cx<R>* twl) {
   static_assert((P & (P - 1)) == 0, "power of two");
   static_assert(P / T >= 2, "need at least two elements per thread");
-  chain<R, P, T, 1, INV, LOG2N, (1 << LOG2N) / P, MAXR>(buf, twl);
+  chain<R, P, T, 1, INV, LOG2N, (1 << LOG2N) / P, MAXR>(buf, twl, threadIdx.x);
+}
+
+// G independent P-point transforms by one workgroup of G*TG threads: threads
+// [g*TG, (g+1)*TG) transform the padded buffer buf + g*padded_size(P).  Every
+// pass synchronises the whole workgroup, so all threads must call this.
+template <typename R, int P, int TG, int G, bool INV, int LOG2N, int MAXR = 32>
+__device__ __forceinline__ void fft_multi(cx<R>* buf, const cx<R>* twl) {
+  static_assert((P & (P - 1)) == 0, "power of two");
+  static_assert(P / TG >= 2, "need at least two elements per thread");
+  const int g = threadIdx.x / TG;
+  chain<R, P, TG, 1, INV, LOG2N, (1 << LOG2N) / P, MAXR>(buf + g * padded_size(P), twl, threadIdx.x % TG);
 }
 
 }  // namespace rksfft

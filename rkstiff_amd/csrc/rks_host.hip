@@ -9,6 +9,7 @@
 #include <string>
 #include <vector>
 
+#include "rks_2d_host.h"
 #include "rks_kernels.h"
 
 // ---------------------------------------------------------------------------
@@ -70,7 +71,7 @@ __global__ __launch_bounds__(1024) void rks_norm_kernel(const cx<R>* u, const cx
     }
   }
   wg_sum2<1024>(su, se, red);
-  if (threadIdx.x == 0) *s_out = safety * pow(eps * sqrt(su) / sqrt(se), 1.0 / (double)q);
+  if (threadIdx.x == 0) *s_out = rks_error_ratio(su, se, eps, safety, q, sizeof(R) < sizeof(double));
 }
 
 __global__ void rks_zero_int(int* p, int n) {
@@ -254,6 +255,9 @@ struct rks_plan {
   std::vector<int> bucket_log2, bucket_start, bucket_count;
   std::vector<int> order;
   int snap_cap = 0, store_freq = 0, log_cap = 0;
+  // 2-D rfft2 engine (RKS_NL_AC2D_RFFT2): one grid size for the whole plan
+  rks2_ctx* c2 = nullptr;
+  int log2n2 = 0;
 };
 
 static void fill_ctrl(rks_ctrl* c, const rks_config* cfg);
@@ -276,6 +280,9 @@ int rks_sizeof(int what) {
     case 1: return (int)sizeof(rks_inst);
     case 2: return (int)sizeof(rks_io);
     case 3: return (int)sizeof(rks_ctrl);
+    case 4: return (int)sizeof(rks_instance_desc);
+    case 5: return (int)sizeof(rks_config);
+    case 6: return (int)sizeof(rks_stats);
     default: return -1;
   }
 }
@@ -288,8 +295,10 @@ int rks_plan_create(int method, int nl_kind, int precision, const rks_config* cf
     return fail(RKS_ERR_INVALID, "precision must be RKS_F64, RKS_F32 or RKS_MIXED");
   if (precision == RKS_MIXED && nl_kind == RKS_NL_NONE)
     return fail(RKS_ERR_INVALID, "mixed precision needs a built-in nonlinearity");
-  if (nl_kind != RKS_NL_ADVECT_RFFT && nl_kind != RKS_NL_CUBIC_FFT && nl_kind != RKS_NL_NONE)
+  if (nl_kind != RKS_NL_ADVECT_RFFT && nl_kind != RKS_NL_CUBIC_FFT && nl_kind != RKS_NL_NONE &&
+      nl_kind != RKS_NL_AC2D_RFFT2)
     return fail(RKS_ERR_INVALID, "unknown nonlinear kind");
+  const bool is2d = nl_kind == RKS_NL_AC2D_RFFT2;
   rks_plan* p = new rks_plan();
   if (rks_build_tableau(method, &p->tab)) {
     delete p;
@@ -339,6 +348,18 @@ int rks_plan_create(int method, int nl_kind, int precision, const rks_config* cf
         return fail(RKS_ERR_INVALID, "rfft grid size must be a power of two in [16, 8192]");
       }
       m = n / 2 + 1;
+    } else if (is2d) {
+      const int l2 = ilog2(n);
+      if (desc[i].ny != n || !rks2_supported(precision, l2)) {
+        rks_plan_destroy(p);
+        return fail(RKS_ERR_INVALID, "rfft2 grids must be square, N x N with N a power of two in [32, 512]");
+      }
+      if (i > 0 && desc[i].n != desc[0].n) {
+        rks_plan_destroy(p);
+        return fail(RKS_ERR_INVALID, "all trajectories of an rfft2 plan share one grid size");
+      }
+      m = rks2_modes(l2);
+      p->log2n2 = l2;
     } else if (nl_kind == RKS_NL_CUBIC_FFT) {
       const int l2 = ilog2(n);
       if (l2 < 4 || l2 > 13) {
@@ -365,6 +386,7 @@ int rks_plan_create(int method, int nl_kind, int precision, const rks_config* cf
     p->ws_off[i] = ws_total;
     ws_total += rks_align256((size_t)p->tab.nslot * m * p->nsize) + 2 * rks_align256((size_t)m * p->esize) +
                 rks_align256(p->coef_bytes_per_mode * m);
+    if (is2d) ws_total += rks_align256((size_t)m * p->esize);  // transform buffer T
     Loff[i] = L_total;
     L_total += (size_t)m * (desc[i].L_is_complex ? 2 : 1);
     kxoff[i] = kx_total;
@@ -417,8 +439,29 @@ int rks_plan_create(int method, int nl_kind, int precision, const rks_config* cf
     for (int s = 0; s < RKS_MAX_SLOTS; ++s) r.slot_map[s] = s;
   }
   HIPCHK(hipMemcpy(p->d_inst, p->h_inst.data(), sizeof(rks_inst) * ninst, hipMemcpyHostToDevice));
+  if (is2d) {
+    const int l = p->log2n2, N = 1 << l, m = p->modes[0];
+    HIPCHK(hipMalloc(&p->d_tw[l], (size_t)N * p->esize));
+    if (precision != RKS_F32)
+      hipLaunchKernelGGL(rks_twiddle_kernel<double>, dim3((N + 255) / 256), dim3(256), 0, 0, (cx<double>*)p->d_tw[l],
+                         N);
+    else
+      hipLaunchKernelGGL(rks_twiddle_kernel<float>, dim3((N + 255) / 256), dim3(256), 0, 0, (cx<float>*)p->d_tw[l],
+                         N);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipDeviceSynchronize());
+    const size_t off_c0 = rks_align256((size_t)p->tab.nslot * m * p->nsize);
+    const size_t off_c1 = off_c0 + rks_align256((size_t)m * p->esize);
+    const size_t off_cf = off_c1 + rks_align256((size_t)m * p->esize);
+    const size_t off_T = off_cf + rks_align256(p->coef_bytes_per_mode * m);
+    if (rks2_create(&p->c2, precision, l, ninst, p->d_tab, p->d_ctl, p->d_inst, p->d_io, p->d_tw[l], off_c0, off_c1,
+                    off_cf, off_T)) {
+      rks_plan_destroy(p);
+      return fail(RKS_ERR_HIP, "2-D engine set-up failed");
+    }
+  }
   // buckets by size
-  if (nl_kind != RKS_NL_NONE) {
+  if (nl_kind != RKS_NL_NONE && !is2d) {
     std::vector<std::vector<int>> byl(14);
     for (int i = 0; i < ninst; ++i) byl[ilog2(desc[i].n)].push_back(i);
     for (int l = 13; l >= 4; --l) {
@@ -489,6 +532,7 @@ int rks_plan_destroy(rks_plan* p) {
   (void)hipFree(p->d_queue);
   (void)hipFree(p->d_order);
   for (int l = 0; l < 14; ++l) (void)hipFree(p->d_tw[l]);
+  rks2_destroy(p->c2);
   delete p;
   return RKS_OK;
 }
@@ -549,6 +593,11 @@ static char* inst_cbase(const rks_plan* p, int inst) {
 }
 
 static int launch_all(rks_plan* p, int mode, double h_mode, hipStream_t stream) {
+  if (p->c2) {
+    const hipError_t e = rks2_run(p->c2, p->tab, mode, stream);
+    if (e != hipSuccess) return fail(RKS_ERR_HIP, std::string("2-D engine: ") + hipGetErrorString(e));
+    return RKS_OK;
+  }
   rks_launch L;
   L.tab = p->d_tab;
   L.ctl = p->d_ctl;

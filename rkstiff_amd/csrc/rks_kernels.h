@@ -108,6 +108,18 @@ __device__ __forceinline__ void wg_sum2(double& a, double& b, double* red) {
   __syncthreads();
 }
 
+// Error ratio s = safety (eps ||u_new|| / ||err||)^(1/q) from the masked sums
+// (solveras.py:449-455).  With FP32 NL storage (lowp), stages that agree to
+// storage precision -- e.g. a last step of ~1e-15 that lands on tf -- give
+// err == 0 exactly: that error is below what the storage resolves, so s is
+// DBL_MAX (accept, growth capped at MAX_S), not the FP64 zero-error s = inf
+// that the reference rejects.
+__device__ __forceinline__ double rks_error_ratio(double su, double se, double eps, double safety, int q,
+                                                  bool lowp) {
+  if (lowp && se == 0.0 && su > 0.0 && su <= __DBL_MAX__) return __DBL_MAX__;
+  return safety * pow(eps * sqrt(su) / sqrt(se), 1.0 / (double)q);
+}
+
 // ---------------------------------------------------------------------------
 // FP64 coefficients of one mode (etd35.py:169-288 and the other builders)
 // ---------------------------------------------------------------------------
@@ -724,8 +736,7 @@ __device__ void run_instance(const rks_launch& P, int idx, cx<R>* buf, const cx<
         }
       }
       wg_sum2<T>(su, se, red);
-      const double tol = ct.epsilon * sqrt(su);
-      const double sv = ct.safety_f * pow(tol / sqrt(se), 1.0 / (double)tb.q);
+      const double sv = rks_error_ratio(su, se, ct.epsilon, ct.safety_f, tb.q, sizeof(RN) < sizeof(double));
       ++attempts;
       if (t == 0 && log_count < log_cap) {
         in->log[2 * log_count] = h;

@@ -79,6 +79,7 @@ class Plan:
             else:
                 arr[i].kx = None
             arr[i].nl_coef = float(d.get("nl_coef", 1.0))
+            arr[i].ny = int(d.get("ny", 0))
             m = L.numel()
             self.modes.append(m)
         handle = ctypes.c_void_p()

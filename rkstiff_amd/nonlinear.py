@@ -103,6 +103,34 @@ class CubicNL(BuiltinNL):
         return dict(n=self.n, L=L, kx=None, nl_coef=self.gamma)
 
 
+class AllenCahnNL(BuiltinNL):
+    """rfft2(w - coef*w^3), w = irfft2(u), on an n x n periodic grid.
+
+    u is the rfft2 spectrum flattened row-major (n * (n//2 + 1) modes), the
+    layout the reference needs for multi-D problems (solver.py:129-135); the
+    cubic is the Allen-Cahn reaction term of demos/allen_cahn.ipynb cell 4.
+    """
+
+    kind = _lib.NL_AC2D_RFFT2
+
+    def __init__(self, ny, nx=None, coef=1.0):
+        super().__init__()
+        nx = ny if nx is None else nx
+        if int(ny) != int(nx):
+            raise ValueError("the rfft2 nonlinearity needs a square grid (ny == nx)")
+        self.n = int(nx)
+        self.coef = float(coef)
+
+    def n_points(self):
+        return (self.n, self.n)
+
+    def modes(self):
+        return self.n * (self.n // 2 + 1)
+
+    def descriptor(self, L):
+        return dict(n=self.n, ny=self.n, L=L, kx=None, nl_coef=self.coef)
+
+
 # ---------------------------------------------------------------------------
 # model factories: (lin_op, nl_func) pairs, as in the reference's models.py
 # ---------------------------------------------------------------------------
@@ -128,3 +156,12 @@ def nls_ops(kx, gamma=1.0):
     """Cubic NLS u_t = (i/2) u_xx + i gamma |u|^2 u: L = -(i/2) kx^2, N = 1j gamma fft(|v|^2 v)."""
     kx = _as_f64(kx)
     return torch.complex(torch.zeros_like(kx), -0.5 * kx**2), CubicNL(kx.numel(), gamma)
+
+
+def allen_cahn_ops(n, eps=0.01):
+    """2-D Allen-Cahn u_t = eps Lap u + u - u^3 on [0, 2 pi)^2 with integer wavenumbers:
+    L = -eps (kx^2 + ky^2) over the flattened n x (n//2+1) rfft2 modes, N = rfft2(w - w^3)."""
+    ky = torch.fft.fftfreq(n, d=1.0 / n, dtype=torch.float64)
+    kx = torch.fft.rfftfreq(n, d=1.0 / n, dtype=torch.float64)
+    L = -eps * (ky[:, None] ** 2 + kx[None, :] ** 2)
+    return L.reshape(-1), AllenCahnNL(n)

@@ -290,6 +290,8 @@ class BaseSolverAS(BaseSolver):
             self.u.append(u)
         if h_init is None:
             h_init = (tf - t0) / 100.0
+        if not tf > t0:  # no step is taken (solveras.py:624): u comes back unchanged
+            return u
         if self._builtin:
             return self._evolve_engine(u, float(t0), float(tf), float(h_init), store_data,
                                        int(store_freq))
