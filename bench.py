@@ -1,23 +1,33 @@
-"""Benchmark: trajectories/s to tf for the batched FP32 stepping path on one GPU.
+"""Benchmark: trajectories/s to tf on one GPU, batched stepping vs the plain FP64 drop-in.
 
-Workload (default ``nls``): the two-soliton cubic NLS of the reference's NLS
-demo form, u_t = (i/2) u_xx + i |u|^2 u, n = 8192 complex fft grid on
-[-64, 64), L = -(i/2) kx^2, ETD35 adaptive with epsilon = 1e-6, t0 = 0, tf = 20.
-A batch of ``--batch`` trajectories (seeded perturbations of the soliton
-positions and phases) is integrated per step by one launch of the persistent
-stepping kernel: FP32 stages/FFTs, FP64 coefficients, error norm and dt
-controller.  A "step" is one such batch integrated t0 -> tf.
+Workload: the two-soliton cubic NLS, u_t = (i/2) u_xx + i |u|^2 u, n = 8192
+complex fft grid on [-64, 64), L = -(i/2) kx^2, ETD35 adaptive with
+epsilon = 1e-6, t0 = 0, tf = 20.  A batch of ``--batch`` trajectories
+(instance 0 the unperturbed pair, the others seeded shifts of positions and
+phases) is integrated by one launch of the persistent stepping kernel; a
+"step" is that batch integrated t0 -> tf, inputs resident in HBM.
+
+Precision modes (all with FP64 coefficient generation, error norm and dt
+control on the device):
+  f64    FP64 throughout (the parity path)
+  mixed  FP64 state, propagators and FFTs; FP32 NL arrays and the other
+         coefficients
+  f32    FP32 state, stages and FFTs
+The headline value is the fastest mode whose max rel L2 error at tf against
+the FP64 engine is <= 1e-6 (the error is in the line).
 
 Reported beside the value:
   * the plain per-instance FP64 drop-in (one ETD35.evolve per trajectory),
-  * rel L2 error at tf of every FP32 trajectory against the FP64 engine,
-  * the HBM roofline of the stepping kernel (algorithmic bytes per launch
-    from the per-attempt byte model, divided by the launch time measured with
-    HIP events on the launch stream),
+  * the HBM roofline of the stepping kernel: algorithmic bytes per launch
+    (per-attempt byte model below) / launch time from HIP events recorded on
+    the launch stream,
   * the CPU baseline: the NumPy FP64 oracle on a bounded sample of the same
-    workload on the host cores.
+    workload, one trajectory per host core.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--batch B]
+                       [--modes f64,mixed,f32] [--plain P] [--no-cpu]
+                       [--copy-mib X]   (first run a device copy of X MiB:
+                                         the reference point for HBM counters)
 """
 import argparse
 import json
@@ -72,6 +82,10 @@ def bytes_per_accept(m, e_u, e_nl):
     return m * (e_u + e_nl)
 
 
+# HBM bytes per trajectory of this workload measured with rocprofv3 FETCH_SIZE / WRITE_SIZE
+# (scaled by a copy of known size; profiles/rks_kernel_nls8192_etd35_b256_hbm_counters.txt)
+TRAFFIC_PER_TRAJECTORY = {"f64": 1.445e10, "mixed": 7.58e9}
+
 MODES = {  # name: (precision id, state element bytes, NL element bytes)
     "f32": ("F32", 8, 8),
     "mixed": ("MIXED", 16, 8),
@@ -114,6 +128,8 @@ def main():
     ap.add_argument("--plain", type=int, default=2, help="trajectories for the plain FP64 figure")
     ap.add_argument("--cpu-workers", type=int, default=0, help="0: min(16, cpu_count)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--modes", default="f64,mixed,f32", help="precision modes to run (f64 always runs)")
+    ap.add_argument("--copy-mib", type=int, default=0, help="device copy of this size first (counter scale)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; trajectories are independent, "
@@ -169,12 +185,21 @@ def main():
         del plan
         return out, uf
 
+    if args.copy_mib > 0:
+        src = torch.empty(args.copy_mib * 2**18, dtype=torch.float32, device=dev).fill_(1.0)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": src.numel() * 4, "copy_bytes_written": dst.numel() * 4}),
+              flush=True)
+        del src, dst
+    modes = ["f64"] + [m for m in args.modes.split(",") if m in MODES and m != "f64"]
     results = {}
     finals = {}
-    for name in ("f64", "mixed", "f32"):
+    for name in modes:
         results[name], finals[name] = run_mode(name)
     ref = finals["f64"]
-    for name in ("mixed", "f32"):
+    for name in modes[1:]:
         errs = np.array([float(torch.linalg.norm(finals[name][i].to(torch.complex128) - ref[i])
                                / torch.linalg.norm(ref[i])) for i in range(B)])
         results[name]["rel_l2_max"] = float(errs.max())
@@ -185,15 +210,17 @@ def main():
     head = max(eligible, key=lambda k: results[k]["value"])
     hr = results[head]
 
-    plain = ETD35(torch.as_tensor(L), nl, config=cfg)
-    u064 = [torch.as_tensor(u, dtype=torch.complex128, device=dev) for u in u0_host[:args.plain]]
-    plain.evolve(u064[0], 0.0, 1.0, store_data=False)
-    torch.cuda.synchronize()
-    tp = time.perf_counter()
-    for i in range(args.plain):
-        plain.evolve(u064[i], 0.0, TF, store_data=False)
-    torch.cuda.synchronize()
-    plain_s = (time.perf_counter() - tp) / args.plain
+    plain_s = None
+    if args.plain > 0:
+        plain = ETD35(torch.as_tensor(L), nl, config=cfg)
+        u064 = [torch.as_tensor(u, dtype=torch.complex128, device=dev) for u in u0_host[:args.plain]]
+        plain.evolve(u064[0], 0.0, 1.0, store_data=False)
+        torch.cuda.synchronize()
+        tp = time.perf_counter()
+        for i in range(args.plain):
+            plain.evolve(u064[i], 0.0, TF, store_data=False)
+        torch.cuda.synchronize()
+        plain_s = (time.perf_counter() - tp) / args.plain
 
     line = {
         "metric": METRIC,
@@ -215,15 +242,17 @@ def main():
                                  "f32": "f32 stages and FFTs; f64 coefficients, norm, dt control"}[head],
                    "parallelism": "1 GPU, one workgroup per trajectory, persistent kernel"},
         "roofline": {"bound": "hbm", "achieved": round(hr["achieved_GBs"], 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(hr["achieved_GBs"] / HBM_PEAK_GBS, 4), "traffic": None},
+                     "unit": "GB/s", "frac": round(hr["achieved_GBs"] / HBM_PEAK_GBS, 4),
+                     "traffic": (None if head not in TRAFFIC_PER_TRAJECTORY
+                                 else float(f"{TRAFFIC_PER_TRAJECTORY[head] * B:.4g}"))},
         "accuracy": {"rel_l2_vs_fp64_max": hr["rel_l2_max"], "rel_l2_vs_fp64_median": hr["rel_l2_median"],
                      "gate": 1e-6},
         "modes": {k: {kk: (round(vv, 6) if isinstance(vv, float) else vv) for kk, vv in v.items()}
                   for k, v in results.items()},
         "kernel_ms_per_launch": round(hr["kernel_ms"], 3),
-        "plain_fp64": {"value": round(1.0 / plain_s, 4), "unit": "trajectories/s",
-                       "s_per_trajectory": round(plain_s, 4)},
-        "batched_over_plain": round(hr["value"] * plain_s, 2),
+        "plain_fp64": None if plain_s is None else {"value": round(1.0 / plain_s, 4), "unit": "trajectories/s",
+                                                    "s_per_trajectory": round(plain_s, 4)},
+        "batched_over_plain": None if plain_s is None else round(hr["value"] * plain_s, 2),
     }
     if not args.no_cpu:
         workers = args.cpu_workers or min(16, os.cpu_count() or 1)
