@@ -411,6 +411,70 @@ __device__ __forceinline__ void error_chunk(const rks_tableau& tb, SlotF slotp, 
   }
 }
 
+// The last stage and the embedded error read the same NL slots in the same
+// order with scalar error weights (ETD35: k = E u + sum c_j NL_j,
+// err = a75 (-NL1 + 4 NL3 - 6 NL4 + 4 NL5 - NL6), etd35.py:340-344): one pass
+// forms both from a single load of each NL value.  Arithmetic per quantity is
+// that of stage_chunk / error_chunk.
+__device__ __forceinline__ bool rks_fused_final(const rks_tableau& tb) {
+  if (!tb.adaptive || !tb.err_factored) return false;
+  const rks_stage& S = tb.st[tb.nstage - 1];
+  if (S.has_nl || S.hsum || S.nterm != tb.nerr) return false;
+  for (int j = 0; j < tb.nerr; ++j) {
+    const rks_term &a = S.t[j], &e = tb.err[j];
+    if (a.src != e.src || a.src2 >= 0 || e.src2 >= 0 || e.coef >= 0 || a.hmode != 0 || e.hmode != 0) return false;
+  }
+  return true;
+}
+
+template <typename RS, typename RN, int CH, typename SlotF>
+__device__ __forceinline__ void final_chunk(const rks_tableau& tb, const cx<RS>* __restrict__ u, SlotF slotp,
+                                            const char* cbase, size_t M, int m0, int T, int mlim, double h,
+                                            cx<RS> (&acc)[CH], cx<RS> (&ev)[CH]) {
+  using C = cx<RS>;
+  const rks_stage& S = tb.st[tb.nstage - 1];
+  const bool he = S.ecoef >= 0;
+  const CoefPtr<RS, RN> ec = he ? coef_ptr<RS, RN>(tb, cbase, M, S.ecoef) : CoefPtr<RS, RN>{nullptr, nullptr};
+#pragma unroll
+  for (int i = 0; i < CH; ++i) {
+    const int m = m0 + i * T;
+    acc[i] = mkc((RS)0, (RS)0);
+    ev[i] = acc[i];
+    if (m < mlim) acc[i] = he ? ec[m] * u[m] : u[m];
+  }
+  for (int j = 0; j < S.nterm; ++j) {
+    const rks_term& tm = S.t[j];
+    const cx<RN>* nl = slotp(tm.src);
+    const bool hc = tm.coef >= 0;
+    const CoefPtr<RS, RN> cf = hc ? coef_ptr<RS, RN>(tb, cbase, M, tm.coef) : CoefPtr<RS, RN>{nullptr, nullptr};
+    const RS sc = (RS)tm.scale, se = (RS)tb.err[j].scale;
+    const bool unit = tm.scale == 1.0, eunit = tb.err[j].scale == 1.0;
+#pragma unroll
+    for (int i = 0; i < CH; ++i) {
+      const int m = m0 + i * T;
+      if (m < mlim) {
+        const C v = cvt<RS>(nl[m]);
+        C c = hc ? cf[m] * v : v;
+        if (!unit) c = sc * c;
+        acc[i] = acc[i] + c;
+        const C e = eunit ? v : se * v;
+        ev[i] = (j == 0) ? e : ev[i] + e;
+      }
+    }
+  }
+  const bool hc = tb.err_coef >= 0;
+  const CoefPtr<RS, RN> cf = hc ? coef_ptr<RS, RN>(tb, cbase, M, tb.err_coef) : CoefPtr<RS, RN>{nullptr, nullptr};
+#pragma unroll
+  for (int i = 0; i < CH; ++i) {
+    const int m = m0 + i * T;
+    if (m < mlim) {
+      if (tb.err_hpow) ev[i] = (RS)h * ev[i];
+      if (tb.err_scale != 1.0) ev[i] = (RS)tb.err_scale * ev[i];
+      if (hc) ev[i] = cf[m] * ev[i];
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------
 // nonlinear term in LDS.  fill_nl writes the spectrum produced by getk(m)
 // into the padded LDS buffer (inlined into the stage loop); nl_core runs the
@@ -630,6 +694,7 @@ __device__ void run_instance(const rks_launch& P, int idx, cx<R>* buf, const cx<
 
   auto slot = [&](int s) -> CN* { return slot_base + (size_t)((smap >> (4 * s)) & 15u) * M; };
   auto cur_u = [&]() -> const C* { return ubuf < 0 ? uin : (const C*)(ubuf == 0 ? cand0 : cand1); };
+  const bool fused = rks_fused_final(tb);
 
   bool done = false;
   while (!done) {
@@ -675,15 +740,43 @@ __device__ void run_instance(const rks_launch& P, int idx, cx<R>* buf, const cx<
           smap = (smap & ~(15u | (15u << f4))) | b | (a << f4);
         }
       }
+      // max |u_new| is taken where the candidate is formed; with the fused final
+      // stage the per-mode |u_new| and |err|^2 wait in the (idle) LDS buffer
+      double mx = 0.0;
       for (int s = 0; s < tb.nstage; ++s) {
         const rks_stage& S = tb.st[s];
         const bool last = (s == tb.nstage - 1);
+        if (last && fused) {
+          __syncthreads();  // the previous stage's write-back still reads buf
+#pragma unroll 1
+          for (int c = 0; c < Chunk<MPT>::NCH; ++c) {
+            constexpr int CH = Chunk<MPT>::CH;
+            const int m0 = t + c * CH * T;
+            C acc[CH], ev[CH];
+            final_chunk<R, RN, CH>(tb, u, slot, cbase, (size_t)M, m0, T, M, h, acc, ev);
+#pragma unroll
+            for (int i = 0; i < CH; ++i) {
+              const int m = m0 + i * T;
+              if (m < M) {
+                cand[m] = acc[i];
+                const double a = hypot((double)acc[i].re, (double)acc[i].im);
+                const double er = (double)ev[i].re, ei = (double)ev[i].im;
+                mx = nanmax(mx, a);
+                buf[m] = mkc((R)a, (R)(er * er + ei * ei));
+              }
+            }
+          }
+          continue;
+        }
         auto getk = [&](int m0, C(&acc)[Chunk<MPT>::CH]) {
           stage_chunk<R, RN, Chunk<MPT>::CH>(tb, S, u, slot, cbase, (size_t)M, m0, T, M, h, acc);
           if (last) {
 #pragma unroll
             for (int i = 0; i < Chunk<MPT>::CH; ++i)
-              if (m0 + i * T < M) cand[m0 + i * T] = acc[i];
+              if (m0 + i * T < M) {
+                cand[m0 + i * T] = acc[i];
+                if (tb.adaptive) mx = nanmax(mx, hypot((double)acc[i].re, (double)acc[i].im));
+              }
           }
         };
         if (S.has_nl) {
@@ -703,18 +796,22 @@ __device__ void run_instance(const rks_launch& P, int idx, cx<R>* buf, const cx<
         break;
       }
       // ---------------- embedded error norm (solveras.py:449-455) ----------------
-      // pass 1: max |u_new|; pass 2: masked sums of |u_new|^2 and |err|^2 (FP64)
-      double mx = 0.0;
-      for (int i = 0; i < MPT; ++i) {
-        const int m = t + i * T;
-        if (m < M) {
-          const C kv = cand[m];
-          const double a = hypot((double)kv.re, (double)kv.im);
-          mx = (i == 0) ? a : nanmax(mx, a);
-        }
-      }
+      // max |u_new| (taken above); masked sums of |u_new|^2 and |err|^2 (FP64)
       mx = wg_max<T>(mx, red);
       double su = 0.0, se = 0.0;
+      if (fused) {
+        for (int i = 0; i < MPT; ++i) {
+          const int m = t + i * T;
+          if (m < M) {
+            const C v = buf[m];
+            const double a = (double)v.re;
+            if (a / mx > ct.adapt_cutoff) {
+              su += a * a;
+              se += (double)v.im;
+            }
+          }
+        }
+      } else {
 #pragma unroll 1
       for (int c = 0; c < Chunk<MPT>::NCH; ++c) {
         constexpr int CH = Chunk<MPT>::CH;
@@ -734,6 +831,7 @@ __device__ void run_instance(const rks_launch& P, int idx, cx<R>* buf, const cx<
             }
           }
         }
+      }
       }
       wg_sum2<T>(su, se, red);
       const double sv = rks_error_ratio(su, se, ct.epsilon, ct.safety_f, tb.q, sizeof(RN) < sizeof(double));
