@@ -308,107 +308,143 @@ __device__ __attribute__((noinline)) void coef_update(const rks_tableau& tb, con
 // propagator coefficients may be stored in RN (converted on load).
 //   term (rks_term grouping):  inner = NL[src] + scale2 * NL[src2]
 //     hmode 0: scale * (C * inner)   1: ((h * C) * inner) * scale   2: ((scale*h)/den) * inner
+// Every decision that depends on the tableau is uniform and taken once per
+// term, and modes past the end load index 0 (always in bounds) instead of
+// branching, so a chunk's loads are straight-line code and issue back to
+// back; callers discard the lanes with ok[i] false.
 // ---------------------------------------------------------------------------
-template <typename RS, typename RN>
-struct CoefPtr {
-  const cx<RS>* s;
-  const cx<RN>* n;
-  __device__ __forceinline__ cx<RS> operator[](int m) const { return s ? s[m] : cvt<RS>(n[m]); }
+template <int CH>
+struct ChunkIdx {
+  int m[CH];
+  bool ok[CH];
+  __device__ __forceinline__ ChunkIdx(int m0, int T, int mlim) {
+#pragma unroll
+    for (int i = 0; i < CH; ++i) {
+      const int x = m0 + i * T;
+      ok[i] = x < mlim;
+      m[i] = ok[i] ? x : 0;
+    }
+  }
 };
 
-template <typename RS, typename RN>
-__device__ __forceinline__ CoefPtr<RS, RN> coef_ptr(const rks_tableau& tb, const char* cbase, size_t M, int c) {
-  CoefPtr<RS, RN> p;
+template <typename RS, typename RN, int CH>
+__device__ __forceinline__ void load_coef(const rks_tableau& tb, const char* cbase, size_t M, int c,
+                                          const ChunkIdx<CH>& ix, cx<RS> (&w)[CH]) {
   const char* q = cbase + (size_t)tb.coef_off[c] * M;
-  p.s = tb.coef_rs[c] ? (const cx<RS>*)q : nullptr;
-  p.n = tb.coef_rs[c] ? nullptr : (const cx<RN>*)q;
-  return p;
+  if (tb.coef_rs[c]) {
+    const cx<RS>* p = (const cx<RS>*)q;
+#pragma unroll
+    for (int i = 0; i < CH; ++i) w[i] = p[ix.m[i]];
+  } else {
+    const cx<RN>* p = (const cx<RN>*)q;
+#pragma unroll
+    for (int i = 0; i < CH; ++i) w[i] = cvt<RS>(p[ix.m[i]]);
+  }
 }
 
 // slotp(s): NL array of slot s (const cx<RN>*)
 template <typename RS, typename RN, int CH, bool HSUM, typename SlotF>
 __device__ __forceinline__ void add_terms(const rks_tableau& tb, const rks_term* terms, int nterm, SlotF slotp,
-                                          const char* cbase, size_t M, int m0, int T, int mlim, double h,
+                                          const char* cbase, size_t M, const ChunkIdx<CH>& ix, double h,
                                           cx<RS> (&acc)[CH]) {
   using C = cx<RS>;
   for (int j = 0; j < nterm; ++j) {
     const rks_term& tm = terms[j];
     const cx<RN>* nl = slotp(tm.src);
-    const cx<RN>* nl2 = tm.src2 >= 0 ? slotp(tm.src2) : nullptr;
-    const bool hc = tm.coef >= 0;
-    const CoefPtr<RS, RN> cf = hc ? coef_ptr<RS, RN>(tb, cbase, M, tm.coef) : CoefPtr<RS, RN>{nullptr, nullptr};
-    const RS s2 = (RS)tm.scale2;
-    const RS hr = (RS)h;
-    const RS sc = (RS)tm.scale;
-    const RS hs = (RS)((tm.scale * h) / tm.den);
-    const bool unit = tm.scale == 1.0;
+    C v[CH];
 #pragma unroll
-    for (int i = 0; i < CH; ++i) {
-      const int m = m0 + i * T;
-      if (m < mlim) {
-        C v = cvt<RS>(nl[m]);
-        if (nl2) v = v + s2 * cvt<RS>(nl2[m]);
-        C c;
-        if (tm.hmode == 0) {
-          c = hc ? cf[m] * v : v;
-          if (!unit) c = sc * c;
-        } else if (tm.hmode == 1) {
-          c = hc ? (hr * cf[m]) * v : hr * v;
-          if (!unit) c = sc * c;
+    for (int i = 0; i < CH; ++i) v[i] = cvt<RS>(nl[ix.m[i]]);
+    if (tm.src2 >= 0) {
+      const cx<RN>* nl2 = slotp(tm.src2);
+      const RS s2 = (RS)tm.scale2;
+#pragma unroll
+      for (int i = 0; i < CH; ++i) v[i] = v[i] + s2 * cvt<RS>(nl2[ix.m[i]]);
+    }
+    if (tm.hmode == 2) {
+      const RS hs = (RS)((tm.scale * h) / tm.den);
+#pragma unroll
+      for (int i = 0; i < CH; ++i) v[i] = hs * v[i];
+    } else {
+      const RS hr = (RS)h;
+      if (tm.coef >= 0) {
+        C w[CH];
+        load_coef<RS, RN, CH>(tb, cbase, M, tm.coef, ix, w);
+        if (tm.hmode == 1) {
+#pragma unroll
+          for (int i = 0; i < CH; ++i) v[i] = (hr * w[i]) * v[i];
         } else {
-          c = hs * v;
+#pragma unroll
+          for (int i = 0; i < CH; ++i) v[i] = w[i] * v[i];
         }
-        acc[i] = (HSUM && j == 0) ? c : acc[i] + c;
+      } else if (tm.hmode == 1) {
+#pragma unroll
+        for (int i = 0; i < CH; ++i) v[i] = hr * v[i];
+      }
+      if (tm.scale != 1.0) {
+        const RS sc = (RS)tm.scale;
+#pragma unroll
+        for (int i = 0; i < CH; ++i) v[i] = sc * v[i];
       }
     }
+#pragma unroll
+    for (int i = 0; i < CH; ++i) acc[i] = (HSUM && j == 0) ? v[i] : acc[i] + v[i];
   }
 }
 
-// k = E u + terms, or E u + h * sum(terms) for stages that factor h out (if34.py:126-128)
+// k = E u + terms, or E u + h * sum(terms) for stages that factor h out (if34.py:126-128);
+// lanes past mlim come back zero
 template <typename RS, typename RN, int CH, typename SlotF>
 __device__ __forceinline__ void stage_chunk(const rks_tableau& tb, const rks_stage& S, const cx<RS>* __restrict__ u,
                                             SlotF slotp, const char* cbase, size_t M, int m0, int T, int mlim,
                                             double h, cx<RS> (&acc)[CH]) {
   using C = cx<RS>;
-  const bool he = S.ecoef >= 0;
-  const CoefPtr<RS, RN> ec = he ? coef_ptr<RS, RN>(tb, cbase, M, S.ecoef) : CoefPtr<RS, RN>{nullptr, nullptr};
+  const ChunkIdx<CH> ix(m0, T, mlim);
 #pragma unroll
-  for (int i = 0; i < CH; ++i) {
-    const int m = m0 + i * T;
-    acc[i] = mkc((RS)0, (RS)0);
-    if (m < mlim) acc[i] = he ? ec[m] * u[m] : u[m];
+  for (int i = 0; i < CH; ++i) acc[i] = u[ix.m[i]];
+  if (S.ecoef >= 0) {
+    C e[CH];
+    load_coef<RS, RN, CH>(tb, cbase, M, S.ecoef, ix, e);
+#pragma unroll
+    for (int i = 0; i < CH; ++i) acc[i] = e[i] * acc[i];
   }
   if (S.hsum) {
     C sum[CH];
-    add_terms<RS, RN, CH, true>(tb, S.t, S.nterm, slotp, cbase, M, m0, T, mlim, h, sum);
+    add_terms<RS, RN, CH, true>(tb, S.t, S.nterm, slotp, cbase, M, ix, h, sum);
 #pragma unroll
-    for (int i = 0; i < CH; ++i)
-      if (m0 + i * T < mlim) acc[i] = acc[i] + (RS)h * sum[i];
+    for (int i = 0; i < CH; ++i) acc[i] = acc[i] + (RS)h * sum[i];
   } else {
-    add_terms<RS, RN, CH, false>(tb, S.t, S.nterm, slotp, cbase, M, m0, T, mlim, h, acc);
+    add_terms<RS, RN, CH, false>(tb, S.t, S.nterm, slotp, cbase, M, ix, h, acc);
   }
+#pragma unroll
+  for (int i = 0; i < CH; ++i)
+    if (!ix.ok[i]) acc[i] = mkc((RS)0, (RS)0);
 }
 
 // embedded error estimate (etd35.py:344, etd34.py:190, if34.py:130, if45dp.py:170-177)
 template <typename RS, typename RN, int CH, typename SlotF>
 __device__ __forceinline__ void error_chunk(const rks_tableau& tb, SlotF slotp, const char* cbase, size_t M, int m0,
                                             int T, int mlim, double h, cx<RS> (&ev)[CH]) {
-#pragma unroll
-  for (int i = 0; i < CH; ++i) ev[i] = mkc((RS)0, (RS)0);
-  add_terms<RS, RN, CH, true>(tb, tb.err, tb.nerr, slotp, cbase, M, m0, T, mlim, h, ev);
+  const ChunkIdx<CH> ix(m0, T, mlim);
+  add_terms<RS, RN, CH, true>(tb, tb.err, tb.nerr, slotp, cbase, M, ix, h, ev);
   if (tb.err_factored) {
-    const bool hc = tb.err_coef >= 0;
-    const CoefPtr<RS, RN> cf = hc ? coef_ptr<RS, RN>(tb, cbase, M, tb.err_coef) : CoefPtr<RS, RN>{nullptr, nullptr};
+    if (tb.err_hpow) {
 #pragma unroll
-    for (int i = 0; i < CH; ++i) {
-      const int m = m0 + i * T;
-      if (m < mlim) {
-        if (tb.err_hpow) ev[i] = (RS)h * ev[i];
-        if (tb.err_scale != 1.0) ev[i] = (RS)tb.err_scale * ev[i];
-        if (hc) ev[i] = cf[m] * ev[i];
-      }
+      for (int i = 0; i < CH; ++i) ev[i] = (RS)h * ev[i];
+    }
+    if (tb.err_scale != 1.0) {
+#pragma unroll
+      for (int i = 0; i < CH; ++i) ev[i] = (RS)tb.err_scale * ev[i];
+    }
+    if (tb.err_coef >= 0) {
+      cx<RS> w[CH];
+      load_coef<RS, RN, CH>(tb, cbase, M, tb.err_coef, ix, w);
+#pragma unroll
+      for (int i = 0; i < CH; ++i) ev[i] = w[i] * ev[i];
     }
   }
+#pragma unroll
+  for (int i = 0; i < CH; ++i)
+    if (!ix.ok[i]) ev[i] = mkc((RS)0, (RS)0);
 }
 
 // The last stage and the embedded error read the same NL slots in the same
@@ -433,45 +469,61 @@ __device__ __forceinline__ void final_chunk(const rks_tableau& tb, const cx<RS>*
                                             cx<RS> (&acc)[CH], cx<RS> (&ev)[CH]) {
   using C = cx<RS>;
   const rks_stage& S = tb.st[tb.nstage - 1];
-  const bool he = S.ecoef >= 0;
-  const CoefPtr<RS, RN> ec = he ? coef_ptr<RS, RN>(tb, cbase, M, S.ecoef) : CoefPtr<RS, RN>{nullptr, nullptr};
+  const ChunkIdx<CH> ix(m0, T, mlim);
 #pragma unroll
-  for (int i = 0; i < CH; ++i) {
-    const int m = m0 + i * T;
-    acc[i] = mkc((RS)0, (RS)0);
-    ev[i] = acc[i];
-    if (m < mlim) acc[i] = he ? ec[m] * u[m] : u[m];
+  for (int i = 0; i < CH; ++i) acc[i] = u[ix.m[i]];
+  if (S.ecoef >= 0) {
+    C e[CH];
+    load_coef<RS, RN, CH>(tb, cbase, M, S.ecoef, ix, e);
+#pragma unroll
+    for (int i = 0; i < CH; ++i) acc[i] = e[i] * acc[i];
   }
   for (int j = 0; j < S.nterm; ++j) {
     const rks_term& tm = S.t[j];
     const cx<RN>* nl = slotp(tm.src);
-    const bool hc = tm.coef >= 0;
-    const CoefPtr<RS, RN> cf = hc ? coef_ptr<RS, RN>(tb, cbase, M, tm.coef) : CoefPtr<RS, RN>{nullptr, nullptr};
-    const RS sc = (RS)tm.scale, se = (RS)tb.err[j].scale;
-    const bool unit = tm.scale == 1.0, eunit = tb.err[j].scale == 1.0;
+    C v[CH];
 #pragma unroll
-    for (int i = 0; i < CH; ++i) {
-      const int m = m0 + i * T;
-      if (m < mlim) {
-        const C v = cvt<RS>(nl[m]);
-        C c = hc ? cf[m] * v : v;
-        if (!unit) c = sc * c;
-        acc[i] = acc[i] + c;
-        const C e = eunit ? v : se * v;
-        ev[i] = (j == 0) ? e : ev[i] + e;
+    for (int i = 0; i < CH; ++i) v[i] = cvt<RS>(nl[ix.m[i]]);
+    {
+      C c[CH];
+      if (tm.coef >= 0) {
+        load_coef<RS, RN, CH>(tb, cbase, M, tm.coef, ix, c);
+#pragma unroll
+        for (int i = 0; i < CH; ++i) c[i] = c[i] * v[i];
+      } else {
+#pragma unroll
+        for (int i = 0; i < CH; ++i) c[i] = v[i];
       }
-    }
-  }
-  const bool hc = tb.err_coef >= 0;
-  const CoefPtr<RS, RN> cf = hc ? coef_ptr<RS, RN>(tb, cbase, M, tb.err_coef) : CoefPtr<RS, RN>{nullptr, nullptr};
+      if (tm.scale != 1.0) {
+        const RS sc = (RS)tm.scale;
 #pragma unroll
-  for (int i = 0; i < CH; ++i) {
-    const int m = m0 + i * T;
-    if (m < mlim) {
-      if (tb.err_hpow) ev[i] = (RS)h * ev[i];
-      if (tb.err_scale != 1.0) ev[i] = (RS)tb.err_scale * ev[i];
-      if (hc) ev[i] = cf[m] * ev[i];
+        for (int i = 0; i < CH; ++i) c[i] = sc * c[i];
+      }
+#pragma unroll
+      for (int i = 0; i < CH; ++i) acc[i] = acc[i] + c[i];
     }
+    const double es = tb.err[j].scale;
+    if (es != 1.0) {
+      const RS se = (RS)es;
+#pragma unroll
+      for (int i = 0; i < CH; ++i) v[i] = se * v[i];
+    }
+#pragma unroll
+    for (int i = 0; i < CH; ++i) ev[i] = (j == 0) ? v[i] : ev[i] + v[i];
+  }
+  if (tb.err_hpow) {
+#pragma unroll
+    for (int i = 0; i < CH; ++i) ev[i] = (RS)h * ev[i];
+  }
+  if (tb.err_scale != 1.0) {
+#pragma unroll
+    for (int i = 0; i < CH; ++i) ev[i] = (RS)tb.err_scale * ev[i];
+  }
+  if (tb.err_coef >= 0) {
+    C w[CH];
+    load_coef<RS, RN, CH>(tb, cbase, M, tb.err_coef, ix, w);
+#pragma unroll
+    for (int i = 0; i < CH; ++i) ev[i] = w[i] * ev[i];
   }
 }
 
