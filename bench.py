@@ -1,7 +1,7 @@
 """Benchmark: trajectories/s to tf on one GPU, batched stepping vs the plain FP64 drop-in.
 
-Workload: the two-soliton cubic NLS, u_t = (i/2) u_xx + i |u|^2 u, n = 8192
-complex fft grid on [-64, 64), L = -(i/2) kx^2, ETD35 adaptive with
+Headline workload: the two-soliton cubic NLS, u_t = (i/2) u_xx + i |u|^2 u,
+n = 8192 complex fft grid on [-64, 64), L = -(i/2) kx^2, ETD35 adaptive with
 epsilon = 1e-6, t0 = 0, tf = 20.  A batch of ``--batch`` trajectories
 (instance 0 the unperturbed pair, the others seeded shifts of positions and
 phases) is integrated by one launch of the persistent stepping kernel; a
@@ -16,6 +16,15 @@ control on the device):
 The headline value is the fastest mode whose max rel L2 error at tf against
 the FP64 engine is <= 1e-6 (the error is in the line).
 
+Also measured and reported under "workloads" (one timed pass each):
+  ks_ensemble   4096 Kuramoto-Sivashinsky trajectories, n ~ U{256..4096},
+                domain ~ U[22, 200], u0 = sum of 8 random cosines, IF34 and
+                ETD35, epsilon = 1e-5, tf = 100 (seeded; chaotic, so the FP64
+                sensitivity floor -- u0 rounded through FP32 -- is reported too)
+  allen_cahn_2d 64 trajectories of u_t = 0.01 Lap u + u - u^3 on a 512 x 512
+                periodic grid (rfft2), u0 ~ U[-0.05, 0.05] low-passed to
+                |k| <= 32, ETD35, epsilon = 1e-5, tf = 10
+
 Reported beside the value:
   * the plain per-instance FP64 drop-in (one ETD35.evolve per trajectory),
   * the HBM roofline of the stepping kernel: algorithmic bytes per launch
@@ -26,6 +35,7 @@ Reported beside the value:
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--batch B]
                        [--modes f64,mixed,f32] [--plain P] [--no-cpu]
+                       [--workloads nls,ks,ac2d] [--ks-count C] [--ac-batch A]
                        [--copy-mib X]   (first run a device copy of X MiB:
                                          the reference point for HBM counters)
 """
@@ -43,12 +53,16 @@ sys.path.insert(0, HERE)
 
 METRIC = "trajectories/s to tf at ≤1e-6 rel L2 err vs FP64 (batched vs plain); % HBM peak"
 HBM_PEAK_GBS = 8000.0
+GATE = 1e-6
 
 N = 8192
 TF = 20.0
 EPS = 1e-6
 
 
+# ---------------------------------------------------------------------------
+# synthetic inputs
+# ---------------------------------------------------------------------------
 def nls_initial(n, seed):
     """Two colliding solitons; instance 0 is the unperturbed configuration."""
     dx = 128.0 / n
@@ -66,24 +80,72 @@ def nls_lin_op(n):
     return -0.5j * kx**2
 
 
-def bytes_per_attempt(m, e_u, e_nl, e_L=8):
-    """Algorithmic HBM bytes of one ETD35 attempt (etd35.py:320-344).
+def ks_inputs(count, seed=2024, sizes=(256, 512, 1024, 2048, 4096)):
+    """Per trajectory, in this order: n ~ U{sizes}, domain ~ U[22, 200], a_k ~ N(0, 0.1^2)
+    and theta_k ~ U[0, 2 pi) for k = 1..8; u0 = sum_k a_k cos(2 pi k x / domain + theta_k)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(count):
+        n = int(rng.choice(sizes))
+        dom = float(rng.uniform(22.0, 200.0))
+        amp = rng.normal(0.0, 0.1, 8)
+        th = rng.uniform(0.0, 2 * np.pi, 8)
+        x = np.arange(n) * (dom / n)
+        kx = 2 * np.pi * np.fft.rfftfreq(n, d=dom / n)
+        k = np.arange(1, 9)
+        u0 = np.sum(amp[:, None] * np.cos(2 * np.pi * k[:, None] * x[None, :] / dom + th[:, None]), axis=0)
+        out.append((n, kx, kx**2 * (1 - kx**2), np.fft.rfft(u0)))
+    return out
 
-    Per mode: the six stage combinations read u six times (e_u each) and 19 NL
-    arrays (e_nl), write five NL arrays and the candidate, and read lin_op once
-    per stage (e_L: one FP64 real per mode for the pure-imaginary NLS lin_op).
-    Uniform precision reduces to M [(R_s + W) e_c + S e_L] with R_s=25, W=6, S=6.
-    """
-    return m * (6 * e_u + 19 * e_nl + 5 * e_nl + 1 * e_u + 6 * e_L)
+
+def ac_inputs(count, n=512, seed=7, kcut=32.0):
+    """u0 ~ U[-0.05, 0.05] per point, every mode with |k| > kcut zeroed (integer k on [0, 2 pi)^2)."""
+    rng = np.random.default_rng(seed)
+    ky = np.fft.fftfreq(n, d=1.0 / n)
+    kx = np.fft.rfftfreq(n, d=1.0 / n)
+    mask = np.sqrt(kx[None, :] ** 2 + ky[:, None] ** 2) > kcut
+    out = []
+    for _ in range(count):
+        uh = np.fft.rfft2(rng.uniform(-0.05, 0.05, (n, n)))
+        uh[mask] = 0.0
+        out.append(uh.reshape(-1))
+    return out
+
+
+# ---------------------------------------------------------------------------
+# byte model: algorithmic HBM bytes of one attempt (the stage formulas'
+# reads of u and NL arrays, writes of NL arrays and the candidate, one lin_op
+# read per stage; FFT work stays in LDS, coefficient arrays are not credited)
+# ---------------------------------------------------------------------------
+# method: (u reads, NL reads, NL writes, candidate writes, stages, NL1 = nl(u) after an accept)
+COUNTS = {
+    "ETD35": (6, 19, 5, 1, 6, True),   # etd35.py:320-344
+    "IF34": (4, 7, 4, 1, 4, False),    # if34.py:120-130
+}
+
+
+def attempt_bytes(method, m, e_u, e_nl, e_L=8):
+    u_r, nl_r, nl_w, c_w, s, _ = COUNTS[method]
+    return m * ((u_r + c_w) * e_u + (nl_r + nl_w) * e_nl + s * e_L)
+
+
+def accept_bytes(method, m, e_u, e_nl):
+    return m * (e_u + e_nl) if COUNTS[method][5] else 0
+
+
+def bytes_per_attempt(m, e_u, e_nl, e_L=8):
+    """ETD35 attempt (etd35.py:320-344): 6 u reads, 19 NL reads, 5 NL writes, 1 candidate, 6 L."""
+    return attempt_bytes("ETD35", m, e_u, e_nl, e_L)
 
 
 def bytes_per_accept(m, e_u, e_nl):
     """NL1 = nl(u) after an accepted attempt (etd35.py:317-318): read u, write NL1."""
-    return m * (e_u + e_nl)
+    return accept_bytes("ETD35", m, e_u, e_nl)
 
 
-# HBM bytes per trajectory of this workload measured with rocprofv3 FETCH_SIZE / WRITE_SIZE
-# (scaled by a copy of known size; profiles/rks_kernel_nls8192_etd35_b256_hbm_counters.txt)
+# HBM bytes per trajectory of the headline workload measured with rocprofv3
+# FETCH_SIZE / WRITE_SIZE (scaled by a copy of known size;
+# profiles/rks_kernel_nls8192_etd35_b256_hbm_counters.txt)
 TRAFFIC_PER_TRAJECTORY = {"f64": 1.445e10, "mixed": 7.58e9}
 
 MODES = {  # name: (precision id, state element bytes, NL element bytes)
@@ -91,6 +153,10 @@ MODES = {  # name: (precision id, state element bytes, NL element bytes)
     "mixed": ("MIXED", 16, 8),
     "f64": ("F64", 16, 16),
 }
+
+
+def sig(v, n=4):
+    return float(f"{v:.{n}g}") if isinstance(v, float) else v
 
 
 # ---------------------------------------------------------------------------
@@ -119,6 +185,102 @@ def cpu_baseline(workers):
                       f"wall {wall:.1f} s, single-trajectory {np.median(per):.2f} s"}
 
 
+# ---------------------------------------------------------------------------
+# GPU measurement
+# ---------------------------------------------------------------------------
+def measure(method, descs, nl, u0_host, tf, eps, mode, steps, warmup):
+    """Integrate every trajectory t0 = 0 -> tf in one plan; returns (result dict, final states)."""
+    import torch
+    from rkstiff_amd import _lib
+    from rkstiff_amd.config import SolverConfig
+    from rkstiff_amd.engine import Plan
+    prec, e_u, e_nl = MODES[mode]
+    dt = torch.complex64 if mode == "f32" else torch.complex128
+    B = len(descs)
+    plan = Plan(method, nl, getattr(_lib, prec), SolverConfig(epsilon=eps), None, descs)
+    u0 = [torch.as_tensor(u, dtype=dt, device="cuda") for u in u0_host]
+    uf = [torch.empty_like(u) for u in u0]
+    t0s, tfs, hs = [0.0] * B, [tf] * B, [tf / 100.0] * B
+    stream = torch.cuda.current_stream()
+    for _ in range(warmup):
+        plan.evolve(u0, uf, t0s, tfs, hs)
+    torch.cuda.synchronize()
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    kernel_ms = []
+    t_start = time.perf_counter()
+    for _ in range(steps):
+        ev0.record(stream)
+        plan.evolve(u0, uf, t0s, tfs, hs)
+        ev1.record(stream)
+        ev1.synchronize()
+        kernel_ms.append(ev0.elapsed_time(ev1))
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t_start
+    st = plan.stats()
+    alg = 0.0
+    for s, m in zip(st, plan.modes):
+        alg += s["attempts"] * attempt_bytes(method, m, e_u, e_nl) + s["steps"] * accept_bytes(method, m, e_u, e_nl)
+    k_ms = float(np.mean(kernel_ms))
+    out = {"value": B * steps / wall, "ms_per_step": wall / steps * 1e3, "kernel_ms": k_ms,
+           "achieved_GBs": alg / (k_ms * 1e-3) / 1e9, "attempts": float(np.mean([s["attempts"] for s in st])),
+           "ok": all(s["status"] == 0 for s in st)}
+    del plan
+    return out, uf
+
+
+def rel_errors(a, ref):
+    import torch
+    return np.array([float(torch.linalg.norm(x.to(torch.complex128) - r) / torch.linalg.norm(r))
+                     for x, r in zip(a, ref)])
+
+
+def run_modes(method, descs, nl, u0_host, tf, eps, modes, steps, warmup, floor=False):
+    """f64 first (the reference for the errors), then the other modes."""
+    res, fin = {}, {}
+    for name in modes:
+        res[name], fin[name] = measure(method, descs, nl, u0_host, tf, eps, name, steps, warmup)
+    for name in modes:
+        e = np.zeros(1) if name == "f64" else rel_errors(fin[name], fin["f64"])
+        res[name]["rel_l2_max"] = float(e.max())
+        res[name]["rel_l2_median"] = float(np.median(e))
+    if floor:  # FP64 sensitivity: u0 rounded through FP32, FP64 everywhere else
+        u32 = [np.asarray(u, dtype=np.complex64).astype(np.complex128) for u in u0_host]
+        _, f = measure(method, descs, nl, u32, tf, eps, "f64", 1, 0)
+        e = rel_errors(f, fin["f64"])
+        res["f64_floor"] = {"rel_l2_max": float(e.max()), "rel_l2_median": float(np.median(e)),
+                            "what": "FP64 engine from u0 rounded through FP32 vs FP64 engine"}
+    return res
+
+
+def summarize(res):
+    return {k: {kk: sig(vv) for kk, vv in v.items()} for k, v in res.items()}
+
+
+def ks_workload(args):
+    import torch
+    from rkstiff_amd.nonlinear import AdvectiveNL
+    data = ks_inputs(args.ks_count)
+    nl = AdvectiveNL(torch.as_tensor(data[0][1]), -1.0)
+    descs = [dict(n=n, L=L, kx=kx, nl_coef=-1.0) for n, kx, L, _ in data]
+    u0 = [u for *_, u in data]
+    out = {}
+    for method in ("IF34", "ETD35"):
+        out[method] = summarize(run_modes(method, descs, nl, u0, 100.0, 1e-5, ["f64", "mixed", "f32"], 1, 0,
+                                          floor=(method == "IF34")))
+    return {"count": args.ks_count, "sizes": [256, 512, 1024, 2048, 4096], "tf": 100.0, "epsilon": 1e-5,
+            "unit": "trajectories/s", "results": out}
+
+
+def ac_workload(args):
+    from rkstiff_amd.nonlinear import allen_cahn_ops
+    L, nl = allen_cahn_ops(512)
+    u0 = ac_inputs(args.ac_batch)
+    descs = [nl.descriptor(L) for _ in u0]
+    res = run_modes("ETD35", descs, nl, u0, 10.0, 1e-5, ["f64", "mixed", "f32"], 1, 0)
+    return {"count": args.ac_batch, "grid": [512, 512], "method": "ETD35", "tf": 10.0, "epsilon": 1e-5,
+            "unit": "trajectories/s", "launches": "hipGraph-replayed attempt sequence", "results": summarize(res)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -129,62 +291,23 @@ def main():
     ap.add_argument("--cpu-workers", type=int, default=0, help="0: min(16, cpu_count)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--modes", default="f64,mixed,f32", help="precision modes to run (f64 always runs)")
+    ap.add_argument("--workloads", default="nls,ks,ac2d")
+    ap.add_argument("--ks-count", type=int, default=4096)
+    ap.add_argument("--ac-batch", type=int, default=64)
     ap.add_argument("--copy-mib", type=int, default=0, help="device copy of this size first (counter scale)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; trajectories are independent, "
-                                   "so N GPUs would each run their own share (see DESIGN.md)"}))
+                                   "so N GPUs would each run their own share of them"}))
         sys.exit(2)
 
     import torch
-    from rkstiff_amd import _lib
-    from rkstiff_amd.engine import Plan
     from rkstiff_amd.config import SolverConfig
     from rkstiff_amd.nonlinear import CubicNL
     from rkstiff_amd.solvers import ETD35
 
     torch.cuda.init()
     dev = torch.device("cuda:0")
-    B = args.batch
-    L = nls_lin_op(N)
-    nl = CubicNL(N, 1.0)
-    cfg = SolverConfig(epsilon=EPS)
-    u0_host = [nls_initial(N, i) for i in range(B)]
-    t0s, tfs, hs = [0.0] * B, [TF] * B, [TF / 100.0] * B
-    stream = torch.cuda.current_stream()
-
-    def run_mode(name):
-        prec, e_u, e_nl = MODES[name]
-        dt = torch.complex64 if name == "f32" else torch.complex128
-        plan = Plan("ETD35", nl, getattr(_lib, prec), cfg, None, [nl.descriptor(L) for _ in range(B)])
-        u0 = [torch.as_tensor(u, dtype=dt, device=dev) for u in u0_host]
-        uf = [torch.empty_like(u) for u in u0]
-        for _ in range(args.warmup):
-            plan.evolve(u0, uf, t0s, tfs, hs)
-        torch.cuda.synchronize()
-        ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        kernel_ms = []
-        t_start = time.perf_counter()
-        for _ in range(args.steps):
-            ev0.record(stream)
-            plan.evolve(u0, uf, t0s, tfs, hs)
-            ev1.record(stream)
-            ev1.synchronize()
-            kernel_ms.append(ev0.elapsed_time(ev1))
-        torch.cuda.synchronize()
-        wall = time.perf_counter() - t_start
-        st = plan.stats()
-        attempts = np.array([x["attempts"] for x in st])
-        steps = np.array([x["steps"] for x in st])
-        alg = float(attempts.sum()) * bytes_per_attempt(N, e_u, e_nl) + \
-            float(steps.sum()) * bytes_per_accept(N, e_u, e_nl)
-        k_ms = float(np.mean(kernel_ms))
-        out = {"value": B * args.steps / wall, "ms_per_step": wall / args.steps * 1e3,
-               "kernel_ms": k_ms, "achieved_GBs": alg / (k_ms * 1e-3) / 1e9,
-               "attempts": float(attempts.mean()), "ok": all(x["status"] == 0 for x in st)}
-        del plan
-        return out, uf
-
     if args.copy_mib > 0:
         src = torch.empty(args.copy_mib * 2**18, dtype=torch.float32, device=dev).fill_(1.0)
         dst = torch.empty_like(src)
@@ -193,26 +316,21 @@ def main():
         print(json.dumps({"copy_bytes_read": src.numel() * 4, "copy_bytes_written": dst.numel() * 4}),
               flush=True)
         del src, dst
+
+    B = args.batch
+    L = nls_lin_op(N)
+    nl = CubicNL(N, 1.0)
+    u0_host = [nls_initial(N, i) for i in range(B)]
     modes = ["f64"] + [m for m in args.modes.split(",") if m in MODES and m != "f64"]
-    results = {}
-    finals = {}
-    for name in modes:
-        results[name], finals[name] = run_mode(name)
-    ref = finals["f64"]
-    for name in modes[1:]:
-        errs = np.array([float(torch.linalg.norm(finals[name][i].to(torch.complex128) - ref[i])
-                               / torch.linalg.norm(ref[i])) for i in range(B)])
-        results[name]["rel_l2_max"] = float(errs.max())
-        results[name]["rel_l2_median"] = float(np.median(errs))
-    results["f64"]["rel_l2_max"] = 0.0
-    results["f64"]["rel_l2_median"] = 0.0
-    eligible = [k for k in results if results[k]["rel_l2_max"] <= 1e-6 and results[k]["ok"]]
+    results = run_modes("ETD35", [nl.descriptor(L) for _ in range(B)], nl, u0_host, TF, EPS, modes,
+                        args.steps, args.warmup)
+    eligible = [k for k in modes if results[k]["rel_l2_max"] <= GATE and results[k]["ok"]]
     head = max(eligible, key=lambda k: results[k]["value"])
     hr = results[head]
 
     plain_s = None
     if args.plain > 0:
-        plain = ETD35(torch.as_tensor(L), nl, config=cfg)
+        plain = ETD35(torch.as_tensor(L), nl, config=SolverConfig(epsilon=EPS))
         u064 = [torch.as_tensor(u, dtype=torch.complex128, device=dev) for u in u0_host[:args.plain]]
         plain.evolve(u064[0], 0.0, 1.0, store_data=False)
         torch.cuda.synchronize()
@@ -221,6 +339,13 @@ def main():
             plain.evolve(u064[i], 0.0, TF, store_data=False)
         torch.cuda.synchronize()
         plain_s = (time.perf_counter() - tp) / args.plain
+
+    workloads = {}
+    wl = args.workloads.split(",")
+    if "ks" in wl:
+        workloads["ks_ensemble"] = ks_workload(args)
+    if "ac2d" in wl:
+        workloads["allen_cahn_2d"] = ac_workload(args)
 
     line = {
         "metric": METRIC,
@@ -244,15 +369,15 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(hr["achieved_GBs"], 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(hr["achieved_GBs"] / HBM_PEAK_GBS, 4),
                      "traffic": (None if head not in TRAFFIC_PER_TRAJECTORY
-                                 else float(f"{TRAFFIC_PER_TRAJECTORY[head] * B:.4g}"))},
-        "accuracy": {"rel_l2_vs_fp64_max": hr["rel_l2_max"], "rel_l2_vs_fp64_median": hr["rel_l2_median"],
-                     "gate": 1e-6},
-        "modes": {k: {kk: (round(vv, 6) if isinstance(vv, float) else vv) for kk, vv in v.items()}
-                  for k, v in results.items()},
+                                 else sig(TRAFFIC_PER_TRAJECTORY[head] * B))},
+        "accuracy": {"rel_l2_vs_fp64_max": sig(hr["rel_l2_max"]),
+                     "rel_l2_vs_fp64_median": sig(hr["rel_l2_median"]), "gate": GATE},
+        "modes": summarize(results),
         "kernel_ms_per_launch": round(hr["kernel_ms"], 3),
         "plain_fp64": None if plain_s is None else {"value": round(1.0 / plain_s, 4), "unit": "trajectories/s",
                                                     "s_per_trajectory": round(plain_s, 4)},
         "batched_over_plain": None if plain_s is None else round(hr["value"] * plain_s, 2),
+        "workloads": workloads,
     }
     if not args.no_cpu:
         workers = args.cpu_workers or min(16, os.cpu_count() or 1)
