@@ -128,6 +128,29 @@ struct pick_radix {
                            : (E % 4 == 0 && MAXR >= 4) ? 4 : 2;
 };
 
+// v[r] *= W_N^(r step) (conjugate for INV), r = 1..RAD-1.  FP64: powers of
+// the one table value w = W_N^step by squaring and products p[r] = p[hb]
+// p[r - hb] (hb the highest power of two <= r), at most four multiplications
+// deep, so a butterfly reads two table entries instead of 2 (RAD - 1).  FP32
+// keeps one table lookup per twiddle (the products' rounding would show).
+template <int RAD, bool INV, int LOG2N, typename R>
+__device__ __forceinline__ void apply_twiddles(cx<R> (&v)[RAD], const cx<R>* twl, int step) {
+  if constexpr (sizeof(R) == sizeof(double)) {
+    cx<R> p[RAD];
+    p[1] = twiddle<R, LOG2N>(twl, step);
+#pragma unroll
+    for (int r = 2; r < RAD; ++r) {
+      const int hb = (r >= 8) ? 8 : (r >= 4) ? 4 : (r >= 2) ? 2 : 1;
+      p[r] = (r == hb) ? p[r / 2] * p[r / 2] : p[hb] * p[r - hb];
+    }
+#pragma unroll
+    for (int r = 1; r < RAD; ++r) v[r] = cmul_tw<INV>(v[r], p[r]);
+  } else {
+#pragma unroll
+    for (int r = 1; r < RAD; ++r) v[r] = cmul_tw<INV>(v[r], twiddle<R, LOG2N>(twl, r * step));
+  }
+}
+
 // One Stockham pass.  Twiddle W_{NS*RAD}^{r k} = W_N^{r k (P/(NS RAD)) TSCALE}, where
 // TSCALE = N / P maps a P-point transform onto the N-point table.
 // t: the thread's index among the T threads working on this transform.
@@ -147,17 +170,52 @@ __device__ __forceinline__ void pass(cx<R>* buf, const cx<R>* twl, int t) {
   for (int b = 0; b < NB; ++b) {
     const int j = t + b * T;
     const int k = j & (NS - 1);
-    if (NS > 1) {
-      const int step = k * (P / (NS * RAD)) * TSCALE;
-#pragma unroll
-      for (int r = 1; r < RAD; ++r) v[b][r] = cmul_tw<INV>(v[b][r], twiddle<R, LOG2N>(twl, r * step));
-    }
+    if (NS > 1) apply_twiddles<RAD, INV, LOG2N>(v[b], twl, k * (P / (NS * RAD)) * TSCALE);
     dft_small<RAD, INV>(v[b]);
     const int base = (j - k) * RAD + k;
 #pragma unroll
     for (int r = 0; r < RAD; ++r) buf[pad(base + r * NS)] = v[b][r];
   }
   __syncthreads();
+}
+
+// The last pass of a transform with its outputs handed to post(x, value)
+// (x the natural-order index) instead of written to LDS.
+template <typename R, int P, int T, int NS, int RAD, bool INV, int LOG2N, int TSCALE, typename Post>
+__device__ __forceinline__ void last_pass(const cx<R>* buf, const cx<R>* twl, int t, Post post) {
+  constexpr int NB = P / (RAD * T);
+  static_assert(NB >= 1, "too few elements per thread");
+  static_assert(NS * RAD == P, "not the last pass");
+  cx<R> v[NB][RAD];
+#pragma unroll
+  for (int b = 0; b < NB; ++b) {
+    const int j = t + b * T;
+#pragma unroll
+    for (int r = 0; r < RAD; ++r) v[b][r] = buf[pad(j + r * (P / RAD))];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int b = 0; b < NB; ++b) {
+    const int j = t + b * T;  // j < NS: the butterfly's outputs land at j + r*NS
+    if (NS > 1) apply_twiddles<RAD, INV, LOG2N>(v[b], twl, j * (P / (NS * RAD)) * TSCALE);
+    dft_small<RAD, INV>(v[b]);
+#pragma unroll
+    for (int r = 0; r < RAD; ++r) post(j + r * NS, v[b][r]);
+  }
+}
+
+template <typename R, int P, int T, int NS, bool INV, int LOG2N, int TSCALE, int MAXR, typename Post>
+__device__ __forceinline__ void chain_post(cx<R>* buf, const cx<R>* twl, int t, Post post) {
+  constexpr int E = P / T;
+  constexpr int R0 = pick_radix<E, MAXR>::v;
+  constexpr int REM = P / NS;
+  constexpr int RAD = (R0 <= REM) ? R0 : REM;
+  if constexpr (NS * RAD == P) {
+    last_pass<R, P, T, NS, RAD, INV, LOG2N, TSCALE>(buf, twl, t, post);
+  } else {
+    pass<R, P, T, NS, RAD, INV, LOG2N, TSCALE>(buf, twl, t);
+    chain_post<R, P, T, NS * RAD, INV, LOG2N, TSCALE, MAXR>(buf, twl, t, post);
+  }
 }
 
 template <typename R, int P, int T, int NS, bool INV, int LOG2N, int TSCALE, int MAXR>
@@ -179,6 +237,17 @@ __device__ __forceinline__ void fft(cx<R>* buf, const cx<R>* twl) {
   static_assert((P & (P - 1)) == 0, "power of two");
   static_assert(P / T >= 2, "need at least two elements per thread");
   chain<R, P, T, 1, INV, LOG2N, (1 << LOG2N) / P, MAXR>(buf, twl, threadIdx.x);
+}
+
+// Full P-point transform whose last pass hands each output (natural-order
+// index x, value) to post instead of storing it: post may write LDS (only the
+// index it is given) or global memory.  Ends after the last pass's reads are
+// synchronised; callers synchronise before reading what post wrote to LDS.
+template <typename R, int P, int T, bool INV, int LOG2N, int MAXR, typename Post>
+__device__ __forceinline__ void fft_post(cx<R>* buf, const cx<R>* twl, Post post) {
+  static_assert((P & (P - 1)) == 0, "power of two");
+  static_assert(P / T >= 2, "need at least two elements per thread");
+  chain_post<R, P, T, 1, INV, LOG2N, (1 << LOG2N) / P, MAXR>(buf, twl, threadIdx.x, post);
 }
 
 // G independent P-point transforms by one workgroup of G*TG threads: threads

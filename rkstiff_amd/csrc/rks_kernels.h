@@ -589,22 +589,16 @@ __device__ __attribute__((noinline)) void nl_core(cx<R>* buf, const cx<R>* twl, 
   const R inv = (R)1 / (R)N;
   __syncthreads();
   if constexpr (NLK == NLK_CUBIC) {
-    rksfft::fft<R, N, T, true, LOG2N, RKS_MAXR>(buf, twl);
-#pragma unroll
-    for (int i = 0; i < N / T; ++i) {
-      const int j = t + i * T;
-      const C v = inv * buf[pad(j)];
+    // |v|^2 v applied as the inverse transform's last pass produces v, and the
+    // forward transform's last pass writes i coef w straight to dst
+    rksfft::fft_post<R, N, T, true, LOG2N, RKS_MAXR>(buf, twl, [&](int x, C w) {
+      const C v = inv * w;
       const R a = v.re * v.re + v.im * v.im;
-      buf[pad(j)] = a * v;
-    }
+      buf[pad(x)] = a * v;
+    });
     __syncthreads();
-    rksfft::fft<R, N, T, false, LOG2N, RKS_MAXR>(buf, twl);
-#pragma unroll
-    for (int i = 0; i < N / T; ++i) {
-      const int m = t + i * T;
-      const C w = buf[pad(m)];
-      dst[m] = mkc((RN)(-coef * w.im), (RN)(coef * w.re));
-    }
+    rksfft::fft_post<R, N, T, false, LOG2N, RKS_MAXR>(
+        buf, twl, [&](int x, C w) { dst[x] = mkc((RN)(-coef * w.im), (RN)(coef * w.re)); });
   } else {
     constexpr int M = N / 2 + 1;
     constexpr int MPT = (M + T - 1) / T;
