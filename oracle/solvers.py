@@ -142,9 +142,10 @@ FIFTH_NAMES = ["a21", "a31", "a32", "a41", "a43", "a51", "a52", "a54",
                "a61", "a62", "a63", "a65", "a71", "a73", "a74", "a75", "a76"]
 
 
-def etd35_coeffs(L, h, etd):
+def etd35_coeffs(L, h, etd, cast=True):
     """ETD(3,5) coefficients, etd35.py:169-288 (contour uses factor*Z)."""
-    L = L.astype(np.complex128, copy=False)
+    if cast:  # the diagonal strategies cast lin_op (etd35.py:124); the diagonalized ones
+        L = L.astype(np.complex128, copy=False)  # keep the eigenvalues' dtype (etd35.py:432)
     z = h * L
     out = {
         "EL14": np.exp(z / 4.0),
@@ -242,9 +243,10 @@ def _fourth_order_rows(p1_12, p2_12, p1, p2, p3):
 FOURTH_NAMES = ["a21", "a31", "a32", "a41", "a43", "a51", "a52", "a54"]
 
 
-def etd34_coeffs(L, h, etd):
+def etd34_coeffs(L, h, etd, cast=True):
     """ETD(3,4) coefficients, etd34.py:96-149; ETD4 uses the same (etd4.py:87-139)."""
-    L = L.astype(np.complex128, copy=False)
+    if cast:  # the diagonal strategies cast lin_op (etd35.py:124); the diagonalized ones
+        L = L.astype(np.complex128, copy=False)  # keep the eigenvalues' dtype (etd35.py:432)
     z = h * L
     out = {"EL": np.exp(z), "EL2": np.exp(z / 2)}
     n = L.shape[0]
@@ -359,7 +361,7 @@ class ETD35(_Method):
     name = "ETD35"
 
     def _build(self, h):
-        return etd35_coeffs(self.L, h, self.etd)
+        return etd35_coeffs(self.L, h, self.etd, cast=not getattr(self, "keep_dtype", False))
 
     def attempt(self, u, h, accept):
         c = self.coeffs(h)
@@ -391,7 +393,7 @@ class ETD34(_Method):
     name = "ETD34"
 
     def _build(self, h):
-        return etd34_coeffs(self.L, h, self.etd)
+        return etd34_coeffs(self.L, h, self.etd, cast=not getattr(self, "keep_dtype", False))
 
     def reset(self):
         super().reset()
@@ -583,6 +585,61 @@ class IF4(_Method):
 METHODS = {m.name: m for m in (ETD35, ETD34, ETD4, ETD5, IF34, IF45DP, IF4)}
 
 
+class Diagonalized:
+    """Diagonalized strategies (etd35.py:348-495, etd34.py:194-304, if34.py:137-196).
+
+    lin_op = S diag(lam) S^-1 (np.linalg.eig); the diagonal scheme runs on
+    v = S^-1 u with the eigenvalues, every stage's NL_k = S^-1 nl(S k); v and
+    NL1 = S^-1 nl(u) are set from u at the first attempt and after an accept
+    (NL1 <- NL5 instead for the FSAL methods).  Returns (S k, err) with err
+    left in the eigenbasis, as the reference does.
+    """
+
+    adaptive = True
+
+    def __init__(self, base, L, nl, etd=None):
+        L = np.asarray(L)
+        if L.ndim != 2:
+            raise ValueError("Cannot diagonalize a 1D system")
+        if np.linalg.cond(L) > 1e16:
+            raise ValueError("Linear operator is non-invertible")
+        self.eig, self.S = np.linalg.eig(L)
+        self.Sinv = np.linalg.inv(self.S)
+        self.user_nl = nl
+        self.m = base(self.eig, self._nl_v, etd)
+        self.m.keep_dtype = True
+        self.q = self.m.q
+        self.name = base.name
+        self.recompute = base is ETD35
+        self.v = None
+
+    def _nl_v(self, k):
+        return self.Sinv.dot(self.user_nl(self.S.dot(k)))
+
+    def reset(self):
+        self.m.reset()
+        self.v = None
+
+    def attempt(self, u, h, accept):
+        m = self.m
+        inner_accept = accept
+        if not m.started:
+            m.NL1 = self.Sinv.dot(self.user_nl(u))
+            self.v = self.Sinv.dot(u)
+            m.started = True
+            inner_accept = False
+        elif accept:
+            self.v = self.Sinv.dot(u)
+            if self.recompute:
+                m.NL1 = self.Sinv.dot(self.user_nl(u))
+                inner_accept = False
+        k, err = m.attempt(self.v, h, inner_accept)
+        return self.S.dot(k), err
+
+
+DIAGONALIZABLE = ("ETD35", "ETD34", "IF34")
+
+
 # ----------------------------------------------------------------------------
 # drivers
 # ----------------------------------------------------------------------------
@@ -721,9 +778,14 @@ class ConstantDriver:
         return u
 
 
-def make_solver(name, L, nl, config=None, etd=None):
-    """Method + driver for ``name`` in METHODS."""
-    m = METHODS[name](L, nl, etd)
+def make_solver(name, L, nl, config=None, etd=None, diagonalize=False):
+    """Method + driver for ``name`` in METHODS (``diagonalize``: a 2-D lin_op in its eigenbasis)."""
+    if diagonalize:
+        if name not in DIAGONALIZABLE:
+            raise ValueError(f"{name} has no diagonalized strategy")
+        m = Diagonalized(METHODS[name], L, nl, etd)
+    else:
+        m = METHODS[name](L, nl, etd)
     if m.adaptive:
         return AdaptiveDriver(m, config)
     return ConstantDriver(m)

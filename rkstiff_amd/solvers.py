@@ -18,8 +18,12 @@ Two paths, both on the device:
   and the error norm run as library kernels, the callable runs between them,
   and the controller runs here with the reference's arithmetic.
 
-Only diagonal (1-D) ``lin_op`` is supported; a square 2-D ``lin_op`` (the
-reference's dense/diagonalized strategies) raises NotImplementedError.
+A square 2-D ``lin_op`` with ``diagonalize=True`` (ETD35, ETD34, IF34) runs
+the reference's diagonalized strategy: lin_op = S diag(lam) S^-1 once on the
+host (as the reference does), the diagonal scheme on v = S^-1 u with the
+eigenvalues through the library kernels, and NL_k = S^-1 nl(S k) with the
+two products on the device.  The dense (``diagonalize=False``) strategy for
+2-D ``lin_op`` raises NotImplementedError.
 """
 import logging
 import math
@@ -52,7 +56,7 @@ class BaseSolver:
 
     method = None
 
-    def __init__(self, lin_op, nl_func, loglevel="WARNING"):
+    def __init__(self, lin_op, nl_func, loglevel="WARNING", diagonalize=False):
         self.lin_op = lin_op
         self.nl_func = nl_func
         self.logger = logging.getLogger(f"rkstiff_amd.{self.__class__.__name__}")
@@ -64,9 +68,13 @@ class BaseSolver:
         if len(dims) == 2 and dims[0] != dims[1]:
             raise ValueError("lin_op must be a square matrix")
         self._diag = len(dims) == 1
+        self._eigen = None
         if not self._diag:
-            raise NotImplementedError("rkstiff_amd steps diagonal (1-D) lin_op only")
-        self._builtin = isinstance(nl_func, BuiltinNL)
+            if not diagonalize:
+                raise NotImplementedError("rkstiff_amd steps diagonal (1-D) lin_op, or a 2-D lin_op with "
+                                          "diagonalize=True (ETD35, ETD34, IF34)")
+            self._eigen = _Eigenbasis(lin_op, self.logger)
+        self._builtin = isinstance(nl_func, BuiltinNL) and self._eigen is None
         self._plans = {}
 
     def set_loglevel(self, loglevel):
@@ -94,7 +102,8 @@ class BaseSolver:
                 desc = self.nl_func.descriptor(self.lin_op)
                 nl = self.nl_func
             else:
-                desc = dict(n=m, L=self.lin_op, kx=None, nl_coef=1.0)
+                L = self.lin_op if self._eigen is None else self._eigen.eig
+                desc = dict(n=m, L=L, kx=None, nl_coef=1.0)
                 nl = None
             p = Plan(self.method, nl, prec, self._config(), self._etd(), [desc],
                      *self._constants())
@@ -161,6 +170,79 @@ class _GenericStages:
         return k, err
 
 
+class _Eigenbasis:
+    """lin_op = S diag(eig) S^-1 (etd35.py:409-421, etd34.py:233-245, if34.py:155-167)."""
+
+    def __init__(self, lin_op, logger):
+        L = lin_op.detach().cpu().numpy() if isinstance(lin_op, torch.Tensor) else np.asarray(lin_op)
+        cond = np.linalg.cond(L)
+        if cond > 1e16:
+            raise ValueError("Linear operator is non-invertible")
+        if cond > 1000:
+            logger.warning(f"Linear matrix array has a large condition number of {cond:.2f}, "
+                           "method may be unstable")
+        self.eig, S = np.linalg.eig(L)
+        self._S = torch.as_tensor(S.astype(np.complex128))
+        self._Sinv = torch.as_tensor(np.linalg.inv(S).astype(np.complex128))
+        self._dev = {}
+
+    def mats(self, like):
+        key = (like.dtype, like.device)
+        if key not in self._dev:
+            self._dev[key] = (self._S.to(device=like.device, dtype=like.dtype),
+                              self._Sinv.to(device=like.device, dtype=like.dtype))
+        return self._dev[key]
+
+
+class _DiagonalizedStages(_GenericStages):
+    """The generic path in the eigenbasis: v = S^-1 u, NL_k = S^-1 nl(S k); returns (S k, err)
+    with err left in the eigenbasis (etd35.py:439-495, etd34.py:258-304, if34.py:175-196)."""
+
+    def __init__(self, method, eigen):
+        super().__init__(method)
+        self.eigen = eigen
+
+    def reset(self):
+        super().reset()
+        self.v = None
+
+    def run(self, plan, nl_func, u, h, accept, adaptive):
+        t = self.t
+        u = u.contiguous()
+        S, Sinv = self.eigen.mats(u)
+
+        def nl_v(k):
+            return torch.matmul(Sinv, self._nl(nl_func, torch.matmul(S, k), k))
+        if h != self.h_coeff:
+            plan.coeffs(0, h)
+            self.h_coeff = h
+        if not self.started:
+            self.slots[self.smap[0]] = torch.matmul(Sinv, self._nl(nl_func, u, u))
+            self.v = torch.matmul(Sinv, u)
+            self.started = True
+        elif accept or not adaptive:
+            self.v = torch.matmul(Sinv, u)
+            if t.nl1_policy == 1:
+                self.slots[self.smap[0]] = torch.matmul(Sinv, self._nl(nl_func, u, u))
+            else:
+                a = self.smap[0]
+                self.smap[0] = self.smap[t.fsal_slot]
+                self.smap[t.fsal_slot] = a
+        v = self.v
+        k = None
+        for s in range(t.nstage):
+            S_ = t.st[s]
+            k = torch.empty_like(v)
+            plan.combine(0, s, h, v, [self.slots[self.smap[j]] for j in range(t.nslot)], k)
+            if S_.has_nl:
+                self.slots[self.smap[S_.nl_slot]] = nl_v(k)
+        err = None
+        if adaptive:
+            err = torch.empty_like(v)
+            plan.combine(0, -1, h, None, [self.slots[self.smap[j]] for j in range(t.nslot)], err)
+        return torch.matmul(S, k), err
+
+
 class BaseSolverAS(BaseSolver):
     """Adaptive-step base: accept/reject controller and evolve (solveras.py:175-650)."""
 
@@ -178,11 +260,16 @@ class BaseSolverAS(BaseSolver):
     MIN_S = 0.25
     SNAP_CHUNK = 128
 
-    def __init__(self, lin_op, nl_func, config=_DEFAULT_CONFIG, loglevel="WARNING"):
-        super().__init__(lin_op, nl_func, loglevel)
+    def __init__(self, lin_op, nl_func, config=_DEFAULT_CONFIG, loglevel="WARNING", diagonalize=False):
+        super().__init__(lin_op, nl_func, loglevel, diagonalize)
         self.config = config
         self._accept = False
-        self._generic = None if self._builtin else _GenericStages(self.method)
+        if self._builtin:
+            self._generic = None
+        elif self._eigen is not None:
+            self._generic = _DiagonalizedStages(self.method, self._eigen)
+        else:
+            self._generic = _GenericStages(self.method)
         self.attempt_log = []
 
     @property
@@ -433,8 +520,8 @@ class BaseSolverCS(BaseSolver):
 
 class ETDAS(BaseSolverAS):
     def __init__(self, lin_op, nl_func, config=_DEFAULT_CONFIG, etd_config=_DEFAULT_ETD,
-                 loglevel="WARNING"):
-        super().__init__(lin_op, nl_func, config, loglevel)
+                 loglevel="WARNING", diagonalize=False):
+        super().__init__(lin_op, nl_func, config, loglevel, diagonalize)
         self.etd_config = etd_config
         self._h_coeff = None
 
@@ -452,7 +539,7 @@ class ETD35(ETDAS):
 
     def __init__(self, lin_op, nl_func, config=_DEFAULT_CONFIG, etd_config=_DEFAULT_ETD,
                  diagonalize=False, loglevel="WARNING"):
-        super().__init__(lin_op, nl_func, config, etd_config, loglevel)
+        super().__init__(lin_op, nl_func, config, etd_config, loglevel, diagonalize)
 
 
 class ETD34(ETDAS):
@@ -461,7 +548,7 @@ class ETD34(ETDAS):
 
     def __init__(self, lin_op, nl_func, config=_DEFAULT_CONFIG, etd_config=_DEFAULT_ETD,
                  diagonalize=False, loglevel="WARNING"):
-        super().__init__(lin_op, nl_func, config, etd_config, loglevel)
+        super().__init__(lin_op, nl_func, config, etd_config, loglevel, diagonalize)
 
 
 class IF34(BaseSolverAS):
@@ -470,7 +557,7 @@ class IF34(BaseSolverAS):
 
     def __init__(self, lin_op, nl_func, config=_DEFAULT_CONFIG, diagonalize=False,
                  loglevel="WARNING"):
-        super().__init__(lin_op, nl_func, config, loglevel)
+        super().__init__(lin_op, nl_func, config, loglevel, diagonalize)
 
 
 class IF45DP(BaseSolverAS):

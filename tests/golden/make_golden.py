@@ -79,12 +79,65 @@ def log_arrays(solver):
     return lg[:, 0], lg[:, 1]
 
 
+def diag_problems():
+    """Two small non-diagonal lin_op matrices (real and complex eigenvalues) with a cubic nl."""
+    rng = np.random.default_rng(11)
+    n = 12
+    S0 = np.eye(n) + 0.25 * rng.standard_normal((n, n))
+    L_real = S0 @ np.diag(-np.linspace(0.5, 6.0, n)) @ np.linalg.inv(S0)
+    u_real = rng.standard_normal(n) + 0j
+    m = 10
+    M = rng.standard_normal((m, m))
+    A = rng.standard_normal((m, m))
+    L_cplx = -(0.1 * M @ M.T + 0.5 * np.eye(m)) + 0.8 * (A - A.T)
+    u_cplx = rng.standard_normal(m) + 1j * rng.standard_normal(m)
+    return {"real": (L_real, u_real, 0.5), "cplx": (L_cplx, u_cplx, 0.2)}
+
+
+def cubic_nl(a):
+    def nl_func(u):
+        return a * u - u * u * u
+    return nl_func
+
+
+def case_diagonalized(rk):
+    """G: diagonalized ETD35 / ETD34 / IF34 (etd35.py:348-495, etd34.py:194-304, if34.py:137-196)."""
+    from rkstiff.solveras import SolverConfig
+    classes = {"ETD35": rk.etd35.ETD35, "ETD34": rk.etd34.ETD34, "IF34": rk.if34.IF34}
+    out = {}
+    for pname, (L, u0, a) in diag_problems().items():
+        out[f"{pname}/L"], out[f"{pname}/u0"], out[f"{pname}/a"] = L, u0, a
+        for mname, cls in classes.items():
+            sv = logged(cls)(lin_op=L, nl_func=cubic_nl(a), config=SolverConfig(epsilon=1e-6),
+                             diagonalize=True)
+            uf = sv.evolve(u0, t0=0.0, tf=3.0, store_data=True, store_freq=5)
+            lh, ls = log_arrays(sv)
+            out[f"{pname}/{mname}/u_final"] = uf
+            out[f"{pname}/{mname}/log_h"], out[f"{pname}/{mname}/log_s"] = lh, ls
+            out[f"{pname}/{mname}/t"] = np.array(sv.t)
+            sv = cls(lin_op=L, nl_func=cubic_nl(a), config=SolverConfig(epsilon=1e-4), diagonalize=True)
+            v, h = u0, 0.2
+            for i in range(3):
+                v, hu, h = sv.step(v, h)
+                out[f"{pname}/{mname}/u{i + 1}"] = v
+                out[f"{pname}/{mname}/hused{i + 1}"] = hu
+                out[f"{pname}/{mname}/hnext{i + 1}"] = h
+    np.savez_compressed(os.path.join(HERE, "diagonalized.npz"), **out)
+    print("G diagonalized done", flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--reference", default="/root/reference")
     ap.add_argument("--skip-nls8192", action="store_true")
+    ap.add_argument("--only", default="", help="comma list of cases to write (default: all)")
     args = ap.parse_args()
     rk = load_reference(args.reference)
+    only = set(filter(None, args.only.split(",")))
+    if only:
+        if "G" in only:
+            case_diagonalized(rk)
+        return
     from rkstiff.solveras import SolverConfig
     from rkstiff.etd import ETDConfig
     grids, models = rk.grids, rk.models
@@ -253,6 +306,8 @@ def main():
         nl256[f"{mname}/u_final"] = sv.evolve(u0, t0=0.0, tf=2.0, h=0.01, store_data=False)
     np.savez_compressed(os.path.join(HERE, "nls256.npz"), **nl256)
     print("F nls256 done", flush=True)
+
+    case_diagonalized(rk)
 
     calib["cpu"] = os.uname().machine
     calib["numpy"] = np.__version__

@@ -348,3 +348,43 @@ def test_snapshot_paging_small_buffer(rk, golden, method):
     np.testing.assert_allclose(solver.t, drv.t, rtol=1e-9)
     for a, b in zip(solver.u, drv.u):
         assert rel(host(a), b) < 1e-10
+
+
+# -- diagonalized strategies (etd35.py:348-495, etd34.py:194-304, if34.py:137-196) --
+@pytest.mark.parametrize("pname", ["real", "cplx"])
+@pytest.mark.parametrize("method", ["ETD35", "ETD34", "IF34"])
+def test_diagonalized_matches_reference(rk, golden, pname, method):
+    """2-D lin_op with diagonalize=True: eigenbasis stages through the library kernels."""
+    g = golden("diagonalized")
+    L, u0, a = g[f"{pname}/L"], g[f"{pname}/u0"], float(g[f"{pname}/a"])
+
+    def nl(u):
+        return a * u - u * u * u
+    solver = getattr(rk, method)(torch.as_tensor(L), nl, config=rk.SolverConfig(epsilon=1e-6), diagonalize=True)
+    uf = solver.evolve(dev(u0), t0=0.0, tf=3.0, store_data=True, store_freq=5)
+    err = rel(host(uf), g[f"{pname}/{method}/u_final"])
+    print(pname, method, "rel", err, "attempts", len(solver.attempt_log))
+    assert len(solver.attempt_log) == len(g[f"{pname}/{method}/log_h"])
+    # step sizes come from s, a ratio of norms of a cancelling stage difference whose
+    # coefficients carry the psi cancellation near modecutoff (the reference computes real
+    # eigenvalues in real arithmetic, the device in complex): ulp differences there move h
+    # by up to ~1e-8 relative, the same bar as the nls8192 attempt log
+    np.testing.assert_allclose([x[0] for x in solver.attempt_log], g[f"{pname}/{method}/log_h"], rtol=1e-6)
+    assert err < 1e-10
+    np.testing.assert_allclose(solver.t, g[f"{pname}/{method}/t"], rtol=1e-8)
+    # three steps from the returned suggestions
+    s2 = getattr(rk, method)(torch.as_tensor(L), nl, config=rk.SolverConfig(epsilon=1e-4), diagonalize=True)
+    v, h = dev(u0), 0.2
+    for i in range(3):
+        v, hu, h = s2.step(v, h)
+        np.testing.assert_allclose([hu, h], [g[f"{pname}/{method}/hused{i + 1}"], g[f"{pname}/{method}/hnext{i + 1}"]],
+                                   rtol=1e-6)
+        assert rel(host(v), g[f"{pname}/{method}/u{i + 1}"]) < 1e-11
+
+
+def test_dense_strategy_not_built(rk):
+    L = torch.eye(4, dtype=torch.float64)
+    with pytest.raises(NotImplementedError):
+        rk.ETD35(L, lambda u: u)
+    with pytest.raises(NotImplementedError):
+        rk.ETD4(L, lambda u: u)

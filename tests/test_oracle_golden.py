@@ -152,3 +152,43 @@ def test_nls256_bitwise(golden, method):
         drv = solvers.make_solver(method, L, problems.nls_nl(1.0))
         uf = drv.evolve(u0, t0=0.0, tf=2.0, h=0.01, store_data=False)
     same(uf, g[f"{method}/u_final"])
+
+
+# -- diagonalized strategies (etd35.py:348-495, etd34.py:194-304, if34.py:137-196) --
+def _cubic(a):
+    return lambda u: a * u - u * u * u
+
+
+@pytest.mark.parametrize("pname", ["real", "cplx"])
+@pytest.mark.parametrize("method", ["ETD35", "ETD34", "IF34"])
+def test_diagonalized_evolve_bitwise(golden, pname, method):
+    g = golden("diagonalized")
+    L, u0, a = g[f"{pname}/L"], g[f"{pname}/u0"], float(g[f"{pname}/a"])
+    drv = solvers.make_solver(method, L, _cubic(a), solvers.SolverConfig(epsilon=1e-6), diagonalize=True)
+    uf = drv.evolve(u0, t0=0.0, tf=3.0, store_data=True, store_freq=5)
+    np.testing.assert_array_equal(uf, g[f"{pname}/{method}/u_final"])
+    np.testing.assert_array_equal([x[0] for x in drv.log], g[f"{pname}/{method}/log_h"])
+    np.testing.assert_array_equal([x[1] for x in drv.log], g[f"{pname}/{method}/log_s"])
+    np.testing.assert_array_equal(drv.t, g[f"{pname}/{method}/t"])
+
+
+@pytest.mark.parametrize("pname", ["real", "cplx"])
+@pytest.mark.parametrize("method", ["ETD35", "ETD34", "IF34"])
+def test_diagonalized_three_steps_bitwise(golden, pname, method):
+    g = golden("diagonalized")
+    L, u0, a = g[f"{pname}/L"], g[f"{pname}/u0"], float(g[f"{pname}/a"])
+    drv = solvers.make_solver(method, L, _cubic(a), solvers.SolverConfig(epsilon=1e-4), diagonalize=True)
+    v, h = u0, 0.2
+    for i in range(3):
+        v, hu, h = drv.step(v, h)
+        np.testing.assert_array_equal(v, g[f"{pname}/{method}/u{i + 1}"])
+        assert hu == g[f"{pname}/{method}/hused{i + 1}"] and h == g[f"{pname}/{method}/hnext{i + 1}"]
+
+
+def test_diagonalized_rejects_bad_operators():
+    with pytest.raises(ValueError):
+        solvers.make_solver("ETD35", np.ones(4), _cubic(0.1), diagonalize=True)
+    with pytest.raises(ValueError):
+        solvers.make_solver("ETD35", np.zeros((3, 3)), _cubic(0.1), diagonalize=True)
+    with pytest.raises(ValueError):
+        solvers.make_solver("IF45DP", np.eye(3), _cubic(0.1), diagonalize=True)
