@@ -22,8 +22,11 @@ A square 2-D ``lin_op`` with ``diagonalize=True`` (ETD35, ETD34, IF34) runs
 the reference's diagonalized strategy: lin_op = S diag(lam) S^-1 once on the
 host (as the reference does), the diagonal scheme on v = S^-1 u with the
 eigenvalues through the library kernels, and NL_k = S^-1 nl(S k) with the
-two products on the device.  The dense (``diagonalize=False``) strategy for
-2-D ``lin_op`` raises NotImplementedError.
+two products on the device.  Without ``diagonalize`` a 2-D ``lin_op`` runs the
+reference's dense strategy (ETD35, ETD34, IF34, ETD4, ETD5, IF4): matrix
+coefficients on the device (matrix exponentials; psi functions by the
+contour integral with one batched inverse per fraction) and the method's
+stages as matrix-vector products.
 """
 import logging
 import math
@@ -69,12 +72,13 @@ class BaseSolver:
             raise ValueError("lin_op must be a square matrix")
         self._diag = len(dims) == 1
         self._eigen = None
+        self._dense = False
         if not self._diag:
-            if not diagonalize:
-                raise NotImplementedError("rkstiff_amd steps diagonal (1-D) lin_op, or a 2-D lin_op with "
-                                          "diagonalize=True (ETD35, ETD34, IF34)")
-            self._eigen = _Eigenbasis(lin_op, self.logger)
-        self._builtin = isinstance(nl_func, BuiltinNL) and self._eigen is None
+            if diagonalize:
+                self._eigen = _Eigenbasis(lin_op, self.logger)
+            else:
+                self._dense = True
+        self._builtin = isinstance(nl_func, BuiltinNL) and self._diag
         self._plans = {}
 
     def set_loglevel(self, loglevel):
@@ -102,7 +106,12 @@ class BaseSolver:
                 desc = self.nl_func.descriptor(self.lin_op)
                 nl = self.nl_func
             else:
-                L = self.lin_op if self._eigen is None else self._eigen.eig
+                if self._eigen is not None:
+                    L = self._eigen.eig
+                elif self._dense:  # the plan only runs the error norm
+                    L = np.zeros(m)
+                else:
+                    L = self.lin_op
                 desc = dict(n=m, L=L, kx=None, nl_coef=1.0)
                 nl = None
             p = Plan(self.method, nl, prec, self._config(), self._etd(), [desc],
@@ -243,6 +252,129 @@ class _DiagonalizedStages(_GenericStages):
         return torch.matmul(S, k), err
 
 
+class _DenseStages(_GenericStages):
+    """Dense strategies (etd35.py:498-720, etd34.py:307-436, etd4.py:178-308, etd5.py:264-482,
+    if34.py:202-254, if4.py:125-199): the method tableau with matrix coefficients.
+
+    Per fraction f of z = h lin_op: E = exp(f z) (matrix exponential) and
+    h psi_r(f z) = h (1/N) sum_m r_m psi_r(r_m) (r_m I - f z)^-1 over the contour
+    r_m = R exp(2 pi i (m + 1/2) / N) (one batched inverse per fraction); each
+    coefficient matrix is its recipe's combination of these.  Stage terms are
+    matrix-vector products."""
+
+    def __init__(self, method, lin_op, etd_config):
+        super().__init__(method)
+        L = lin_op.detach().cpu() if isinstance(lin_op, torch.Tensor) else torch.as_tensor(np.asarray(lin_op))
+        # ETD strategies cast lin_op to complex (etd35.py:553); IF keeps its dtype (if34.py:217)
+        self.L = L.to(torch.complex128)
+        self.etd = etd_config if etd_config is not None else _DEFAULT_ETD
+        self.C = None
+
+    def _psi_weights(self, dev):
+        etd = self.etd
+        npts = int(etd.contour_points)
+        pts = etd.contour_radius * np.exp(2j * np.pi * np.arange(0.5, npts) / npts)
+        e = np.exp(pts)
+        w1 = pts * ((e - 1) / pts) / npts
+        w2 = pts * (2 * (e - 1 - pts) / pts**2) / npts
+        w3 = pts * (6 * (e - 1 - pts - pts**2 / 2) / pts**3) / npts
+        return torch.as_tensor(pts, device=dev), [torch.as_tensor(w, device=dev) for w in (w1, w2, w3)]
+
+    def _coefficients(self, h, like):
+        t = self.t
+        dev = like.device
+        z = (h * self.L).to(dev)
+        n = z.shape[0]
+        eye = torch.eye(n, dtype=torch.complex128, device=dev)
+        pts, ws = self._psi_weights(dev)
+        basis = []
+        for f in range(t.nfrac):
+            fr = t.frac[f]
+            fz = z * (fr.fnum / fr.fden)
+            vals = [torch.linalg.matrix_exp(fz), None, None, None]
+            if fr.mask & 14:
+                Q = torch.linalg.inv(pts[:, None, None] * eye - fz)  # one batched inverse
+                for r in (1, 2, 3):
+                    if fr.mask & (1 << r):
+                        vals[r] = h * torch.einsum("m,mij->ij", ws[r - 1], Q)
+            basis.append(vals)
+        C = []
+        for c in range(t.ncoef):
+            cf = t.coef[c]
+            acc = None
+            for k in range(cf.nb):
+                w = cf.w[k] * h if cf.hmul else cf.w[k]
+                b = basis[cf.frac][cf.kind[k]]
+                term = b if w == 1.0 else w * b
+                acc = term if acc is None else acc + term
+            if cf.outer != 1.0:
+                acc = cf.outer * acc
+            if cf.den != 1.0:
+                acc = acc / cf.den
+            C.append(acc.to(like.dtype))
+        return C
+
+    def _term(self, tm, h, C, slots):
+        v = slots[self.smap[tm.src]]
+        if tm.src2 >= 0:
+            v = v + tm.scale2 * slots[self.smap[tm.src2]]
+        if tm.hmode == 2:
+            return ((tm.scale * h) / tm.den) * v
+        c = v if tm.coef < 0 else torch.matmul(C[tm.coef], v)
+        if tm.hmode == 1:
+            c = h * c
+        return c if tm.scale == 1.0 else tm.scale * c
+
+    def _stage(self, S, u, h, C):
+        acc = u if S.ecoef < 0 else torch.matmul(C[S.ecoef], u)
+        if S.hsum:
+            tot = self._term(S.t[0], h, C, self.slots)
+            for j in range(1, S.nterm):
+                tot = tot + self._term(S.t[j], h, C, self.slots)
+            return acc + h * tot
+        for j in range(S.nterm):
+            acc = acc + self._term(S.t[j], h, C, self.slots)
+        return acc
+
+    def _error(self, h, C):
+        t = self.t
+        ev = self._term(t.err[0], h, C, self.slots)
+        for j in range(1, t.nerr):
+            ev = ev + self._term(t.err[j], h, C, self.slots)
+        if t.err_factored:
+            if t.err_hpow:
+                ev = h * ev
+            if t.err_scale != 1.0:
+                ev = t.err_scale * ev
+            if t.err_coef >= 0:
+                ev = torch.matmul(C[t.err_coef], ev)
+        return ev
+
+    def run(self, plan, nl_func, u, h, accept, adaptive):
+        t = self.t
+        u = u.contiguous()
+        if h != self.h_coeff:
+            self.C = self._coefficients(h, u)
+            self.h_coeff = h
+        if not self.started:
+            self.slots[self.smap[0]] = self._nl(nl_func, u, u)
+            self.started = True
+        elif accept or not adaptive:
+            if t.nl1_policy == 1:
+                self.slots[self.smap[0]] = self._nl(nl_func, u, u)
+            else:
+                a = self.smap[0]
+                self.smap[0] = self.smap[t.fsal_slot]
+                self.smap[t.fsal_slot] = a
+        k = None
+        for s in range(t.nstage):
+            S = t.st[s]
+            k = self._stage(S, u, h, self.C)
+            if S.has_nl:
+                self.slots[self.smap[S.nl_slot]] = self._nl(nl_func, k, u)
+        return k, (self._error(h, self.C) if adaptive else None)
+
+
 class BaseSolverAS(BaseSolver):
     """Adaptive-step base: accept/reject controller and evolve (solveras.py:175-650)."""
 
@@ -268,6 +400,8 @@ class BaseSolverAS(BaseSolver):
             self._generic = None
         elif self._eigen is not None:
             self._generic = _DiagonalizedStages(self.method, self._eigen)
+        elif self._dense:
+            self._generic = _DenseStages(self.method, lin_op, getattr(self, "etd_config", None))
         else:
             self._generic = _GenericStages(self.method)
         self.attempt_log = []
@@ -446,7 +580,12 @@ class BaseSolverCS(BaseSolver):
 
     def __init__(self, lin_op, nl_func, loglevel="WARNING"):
         super().__init__(lin_op, nl_func, loglevel)
-        self._generic = None if self._builtin else _GenericStages(self.method)
+        if self._builtin:
+            self._generic = None
+        elif self._dense:
+            self._generic = _DenseStages(self.method, lin_op, getattr(self, "etd_config", None))
+        else:
+            self._generic = _GenericStages(self.method)
 
     @property
     def solver_type(self):
@@ -521,15 +660,15 @@ class BaseSolverCS(BaseSolver):
 class ETDAS(BaseSolverAS):
     def __init__(self, lin_op, nl_func, config=_DEFAULT_CONFIG, etd_config=_DEFAULT_ETD,
                  loglevel="WARNING", diagonalize=False):
+        self.etd_config = etd_config  # read by the dense strategy's set-up
         super().__init__(lin_op, nl_func, config, loglevel, diagonalize)
-        self.etd_config = etd_config
         self._h_coeff = None
 
 
 class ETDCS(BaseSolverCS):
     def __init__(self, lin_op, nl_func, etd_config=_DEFAULT_ETD, loglevel="WARNING"):
+        self.etd_config = etd_config  # read by the dense strategy's set-up
         super().__init__(lin_op, nl_func, loglevel)
-        self.etd_config = etd_config
         self._h_coeff = None
 
 

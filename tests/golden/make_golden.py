@@ -126,6 +126,43 @@ def case_diagonalized(rk):
     print("G diagonalized done", flush=True)
 
 
+def dense_problems():
+    """Small non-diagonal lin_op matrices whose spectrum times the step sizes stays inside the
+    unit contour of the dense psi evaluation (real and complex eigenvalues)."""
+    rng = np.random.default_rng(23)
+    n = 8
+    S0 = np.eye(n) + 0.3 * rng.standard_normal((n, n))
+    L_real = S0 @ np.diag(-np.linspace(0.05, 0.8, n)) @ np.linalg.inv(S0)
+    u_real = rng.standard_normal(n) + 0j
+    A = rng.standard_normal((n, n))
+    L_cplx = -0.2 * np.eye(n) + 0.15 * (A - A.T) - 0.05 * (A @ A.T) / n
+    u_cplx = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    return {"real": (L_real, u_real, 0.5), "cplx": (L_cplx, u_cplx, 0.3)}
+
+
+def case_dense(rk):
+    """H: dense strategies (etd35.py:498-720, etd34.py:307-436, etd4.py:178-308, etd5.py:264-482,
+    if34.py:202-254, if4.py:125-199)."""
+    from rkstiff.solveras import SolverConfig
+    adaptive = {"ETD35": rk.etd35.ETD35, "ETD34": rk.etd34.ETD34, "IF34": rk.if34.IF34}
+    constant = {"ETD4": rk.etd4.ETD4, "ETD5": rk.etd5.ETD5, "IF4": rk.if4.IF4}
+    out = {}
+    for pname, (L, u0, a) in dense_problems().items():
+        out[f"{pname}/L"], out[f"{pname}/u0"], out[f"{pname}/a"] = L, u0, a
+        for mname, cls in adaptive.items():
+            sv = logged(cls)(lin_op=L, nl_func=cubic_nl(a), config=SolverConfig(epsilon=1e-6))
+            uf = sv.evolve(u0, t0=0.0, tf=2.0, store_data=True, store_freq=4)
+            lh, ls = log_arrays(sv)
+            out[f"{pname}/{mname}/u_final"] = uf
+            out[f"{pname}/{mname}/log_h"], out[f"{pname}/{mname}/log_s"] = lh, ls
+            out[f"{pname}/{mname}/t"] = np.array(sv.t)
+        for mname, cls in constant.items():
+            sv = cls(lin_op=L, nl_func=cubic_nl(a))
+            out[f"{pname}/{mname}/u_final"] = sv.evolve(u0, t0=0.0, tf=1.0, h=0.025, store_data=False)
+    np.savez_compressed(os.path.join(HERE, "dense.npz"), **out)
+    print("H dense done", flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--reference", default="/root/reference")
@@ -137,6 +174,8 @@ def main():
     if only:
         if "G" in only:
             case_diagonalized(rk)
+        if "H" in only:
+            case_dense(rk)
         return
     from rkstiff.solveras import SolverConfig
     from rkstiff.etd import ETDConfig
@@ -308,6 +347,7 @@ def main():
     print("F nls256 done", flush=True)
 
     case_diagonalized(rk)
+    case_dense(rk)
 
     calib["cpu"] = os.uname().machine
     calib["numpy"] = np.__version__

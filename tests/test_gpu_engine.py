@@ -382,9 +382,31 @@ def test_diagonalized_matches_reference(rk, golden, pname, method):
         assert rel(host(v), g[f"{pname}/{method}/u{i + 1}"]) < 1e-11
 
 
-def test_dense_strategy_not_built(rk):
-    L = torch.eye(4, dtype=torch.float64)
-    with pytest.raises(NotImplementedError):
-        rk.ETD35(L, lambda u: u)
-    with pytest.raises(NotImplementedError):
-        rk.ETD4(L, lambda u: u)
+# -- dense strategies (etd35.py:498-720, etd34.py:307-436, etd4.py:178-308, etd5.py:264-482,
+#    if34.py:202-254, if4.py:125-199) --
+@pytest.mark.parametrize("pname", ["real", "cplx"])
+@pytest.mark.parametrize("method", ["ETD35", "ETD34", "IF34", "ETD4", "ETD5", "IF4"])
+def test_dense_matches_reference(rk, golden, pname, method):
+    """2-D lin_op without diagonalize: matrix coefficients and matrix-vector stages on the device.
+    The matrix exponential and inverses are not NumPy/SciPy's algorithms (coefficients agree to
+    ~1e-13).  The embedded error is a fifth difference of O(1) stage values, so those rounding
+    differences move s, and with it every step grown by s > incr_f, by up to ~1e-7 relative
+    (1e-5 on the final clamped step); the state agrees to ~1e-12."""
+    g = golden("dense")
+    L, u0, a = g[f"{pname}/L"], g[f"{pname}/u0"], float(g[f"{pname}/a"])
+
+    def nl(u):
+        return a * u - u * u * u
+    cls = getattr(rk, method)
+    if method in ADAPTIVE:
+        solver = cls(torch.as_tensor(L), nl, config=rk.SolverConfig(epsilon=1e-6))
+        uf = solver.evolve(dev(u0), t0=0.0, tf=2.0, store_data=True, store_freq=4)
+        assert len(solver.attempt_log) == len(g[f"{pname}/{method}/log_h"])
+        np.testing.assert_allclose([x[0] for x in solver.attempt_log], g[f"{pname}/{method}/log_h"], rtol=1e-5)
+        np.testing.assert_allclose(solver.t, g[f"{pname}/{method}/t"], rtol=1e-6)
+    else:
+        solver = cls(torch.as_tensor(L), nl)
+        uf = solver.evolve(dev(u0), t0=0.0, tf=1.0, h=0.025, store_data=False)
+    err = rel(host(uf), g[f"{pname}/{method}/u_final"])
+    print(pname, method, "dense rel", err)
+    assert err < 1e-10

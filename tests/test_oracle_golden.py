@@ -192,3 +192,42 @@ def test_diagonalized_rejects_bad_operators():
         solvers.make_solver("ETD35", np.zeros((3, 3)), _cubic(0.1), diagonalize=True)
     with pytest.raises(ValueError):
         solvers.make_solver("IF45DP", np.eye(3), _cubic(0.1), diagonalize=True)
+
+
+# -- dense strategies (etd35.py:498-720, etd34.py:307-436, etd4.py:178-308, etd5.py:264-482,
+#    if34.py:202-254, if4.py:125-199) --
+@pytest.mark.parametrize("pname", ["real", "cplx"])
+@pytest.mark.parametrize("method", ["ETD35", "ETD34", "IF34"])
+def test_dense_adaptive_bitwise(golden, pname, method):
+    from oracle import dense
+    g = golden("dense")
+    L, u0, a = g[f"{pname}/L"], g[f"{pname}/u0"], float(g[f"{pname}/a"])
+    drv = dense.make_solver(method, L, _cubic(a), solvers.SolverConfig(epsilon=1e-6))
+    uf = drv.evolve(u0, t0=0.0, tf=2.0, store_data=True, store_freq=4)
+    np.testing.assert_array_equal(uf, g[f"{pname}/{method}/u_final"])
+    np.testing.assert_array_equal([x[0] for x in drv.log], g[f"{pname}/{method}/log_h"])
+    np.testing.assert_array_equal(drv.t, g[f"{pname}/{method}/t"])
+
+
+@pytest.mark.parametrize("pname", ["real", "cplx"])
+@pytest.mark.parametrize("method", ["ETD4", "ETD5", "IF4"])
+def test_dense_constant_bitwise(golden, pname, method):
+    from oracle import dense
+    g = golden("dense")
+    L, u0, a = g[f"{pname}/L"], g[f"{pname}/u0"], float(g[f"{pname}/a"])
+    drv = dense.make_solver(method, L, _cubic(a))
+    uf = drv.evolve(u0, t0=0.0, tf=1.0, h=0.025, store_data=False)
+    np.testing.assert_array_equal(uf, g[f"{pname}/{method}/u_final"])
+
+
+def test_dense_matches_diagonalized_on_a_diagonalizable_operator(golden):
+    """Same operator, two strategies: the dense contour psi and the eigenbasis scheme agree."""
+    from oracle import dense
+    g = golden("dense")
+    L, u0, a = g["real/L"], g["real/u0"], float(g["real/a"])
+    cfg = solvers.SolverConfig(epsilon=1e-10)
+    d1 = dense.make_solver("ETD35", L, _cubic(a), cfg)
+    d2 = solvers.make_solver("ETD35", L, _cubic(a), cfg, diagonalize=True)
+    u1 = d1.evolve(u0, t0=0.0, tf=1.0, store_data=False)
+    u2 = d2.evolve(u0, t0=0.0, tf=1.0, store_data=False)
+    assert solvers.rel_l2(u1, u2) < 1e-8
