@@ -184,3 +184,13 @@ def allen_cahn_2d(ny=512, nx=512, eps=0.01, kcut=32.0, seed=0, batch=1):
         u0s.append(uh.reshape(-1))
     return dict(kind="rfft2", ny=ny, nx=nx, L=L, nl=ac2d_nl(ny, nx), u0=u0s,
                 nl_coef=1.0)
+
+
+def cheb_allen_cahn_nl(x):
+    """models.py:240-245: x - (w + x)^3 on the interior points, complex128."""
+    xi = x[1:-1]
+
+    def nl(u):
+        val = xi - np.power(u + xi, 3)
+        return np.asarray(val, dtype=np.complex128).ravel()
+    return nl

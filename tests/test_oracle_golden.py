@@ -231,3 +231,23 @@ def test_dense_matches_diagonalized_on_a_diagonalizable_operator(golden):
     u1 = d1.evolve(u0, t0=0.0, tf=1.0, store_data=False)
     u2 = d2.evolve(u0, t0=0.0, tf=1.0, store_data=False)
     assert solvers.rel_l2(u1, u2) < 1e-8
+
+
+# -- demos/allen_cahn.ipynb: Chebyshev grid, dense and diagonalized strategies --
+@pytest.mark.parametrize("strategy", ["dense", "diag"])
+@pytest.mark.parametrize("method", ["ETD35", "ETD34", "IF34"])
+def test_cheb_allen_cahn_bitwise(golden, strategy, method):
+    from oracle import dense
+    g = golden("cheb_allen_cahn")
+    nl = problems.cheb_allen_cahn_nl(g["x"])
+    cfg = solvers.SolverConfig(epsilon=1e-4)
+    if strategy == "dense":
+        etd = solvers.ETDConfig(contour_points=64, contour_radius=20)
+        drv = dense.make_solver(method, g["L"], nl, cfg, etd)
+    else:
+        etd = solvers.ETDConfig(contour_points=32, contour_radius=1)
+        drv = solvers.make_solver(method, g["L"], nl, cfg, etd, diagonalize=True)
+    uf = drv.evolve(g["w0"], t0=0.0, tf=10.0, store_data=True, store_freq=10)
+    np.testing.assert_array_equal(uf, g[f"{strategy}/{method}/u_final"])
+    np.testing.assert_array_equal([x[0] for x in drv.log], g[f"{strategy}/{method}/log_h"])
+    np.testing.assert_array_equal(drv.t, g[f"{strategy}/{method}/t"])

@@ -90,3 +90,28 @@ def test_tableau_three_steps(golden, pname, method):
         else:
             v = drv.step(v, h0)
         assert solvers.rel_l2(v, g[f"{pname}/{method}/u{i + 1}"]) < 1e-13
+
+
+def test_chebyshev_grid_and_allen_cahn_ops(golden):
+    """grids.construct_x_dx_cheb / models.allen_cahn_ops vs the reference's (grids.py:134-219,
+    models.py:202-247), and D differentiates polynomials of degree <= n exactly."""
+    import torch
+    from rkstiff_amd import grids, models
+    g = golden("cheb_allen_cahn")
+    x, D = grids.construct_x_dx_cheb(20, -1, 1)
+    np.testing.assert_allclose(x.numpy(), g["x"], rtol=0, atol=1e-15)
+    np.testing.assert_allclose(D.numpy(), g["D"], rtol=1e-13, atol=1e-12)
+    L, nl = models.allen_cahn_ops(x, D, 0.01)
+    np.testing.assert_allclose(L.numpy(), g["L"], rtol=1e-12, atol=1e-11)
+    xs, Ds = grids.construct_x_dx_cheb(12, 0.0, 3.0)
+    xs, Ds = xs.numpy(), Ds.numpy()
+    for k in range(1, 12):
+        np.testing.assert_allclose(Ds @ xs**k, k * xs ** (k - 1), rtol=1e-10, atol=1e-9)
+    np.testing.assert_allclose(Ds @ np.ones_like(xs), 0.0, atol=1e-12)
+    w = torch.linspace(-0.5, 0.5, 19, dtype=torch.float64).to(torch.complex128)
+    ref = problems.cheb_allen_cahn_nl(g["x"])(w.numpy())
+    np.testing.assert_allclose(nl(w).numpy(), ref, rtol=1e-14, atol=1e-15)
+    with pytest.raises(TypeError):
+        grids.construct_x_cheb(3.0)
+    with pytest.raises(ValueError):
+        grids.construct_x_cheb(1)
