@@ -148,6 +148,67 @@ int rks_read_log(rks_plan* p, int inst, double* host_hs, int cap, hipStream_t st
  * their times to t_host; marks the buffer consumed. */
 int rks_read_snapshots(rks_plan* p, int inst, void* dst_dev, double* t_host, int count, hipStream_t stream);
 
+/* ---------------------------------------------------------------------------
+ * Learned step-size proposer.  No counterpart in the reference: its controller
+ * keeps h after an accept unless s > incr_f (solveras.py:543-554).  This is an
+ * opt-in addition for batched evolves.  A small MLP (66 -> 256 -> 256 -> 256 -> 1,
+ * GELU, BF16 weights, FP32 accumulation on the matrix cores) proposes the step
+ * after each accepted one.  The embedded error estimate still judges every
+ * attempt, a rejection still uses the reference's rule, and a non-finite output
+ * falls back to the reference's proposal.
+ *
+ * Inputs of an accepted state (raw, RKS_PROPOSER_NIN floats):
+ *   f_b  = 0.5 ln(mean |u_hat|^2 over bin b / n^2 + 1e-30), b < 64.  Mode m has
+ *          wavenumber index w = m (rfft) or min(m, n - m) (fft) and lies in bin
+ *          (64 w) / (n/2 + 1).
+ *   f_64 = ln h of the accepted step.
+ *   f_65 = ln min(s, 1e4) of its attempt.
+ * The network sees (f - mu) / sig.  Its output y estimates ln(h' s' / h), where
+ * (h', s') is the next attempt.  The proposal is
+ * h exp(clamp(y - margin, ln min_s, ln max_s)).
+ *
+ * Parameters: one float32 vector of RKS_PROPOSER_NPARAM values,
+ *   W1 [256][96] (columns 66..95 are zero padding), b1 [256], W2 [256][256], b2,
+ *   W3 [256][256], b3, w4 [256], b4 [1].
+ * A training record (RKS_PROPOSER_REC floats): 66 inputs, the target
+ * ln(h' s' / h), and a valid flag.  The flag is 0 when the next attempt was
+ * clamped to tf or its error was zero or non-finite.
+ * ------------------------------------------------------------------------- */
+enum { RKS_PROPOSER_NIN = 66, RKS_PROPOSER_REC = 68, RKS_PROPOSER_NPARAM = 156673 };
+
+typedef struct rks_mlp rks_mlp;
+
+/* Create on the current device from host params / mu / sig (float32). */
+int rks_mlp_create(const float* params, const float* mu, const float* sig, rks_mlp** out);
+int rks_mlp_destroy(rks_mlp* net);
+/* Synchronise and copy the FP32 parameters to the host. */
+int rks_mlp_get_params(rks_mlp* net, float* params);
+/* nsteps Adam steps (lr, beta 0.9 / 0.999, eps 1e-8) on minibatches of `batch`
+ * (a multiple of 64) device records rec[idx[step * batch + j]], j < batch.
+ * loss_kind 0: pinball loss rho_tau(y - yhat); 1: 0.5 (yhat - y)^2.  Targets
+ * are used as given (clip them beforehand).  loss_dev (optional, device)
+ * receives the batch-mean loss of every step. */
+int rks_mlp_train(rks_mlp* net, const float* rec, int nrec, const int* idx, int batch, int nsteps, int loss_kind,
+                  double tau, double lr, float* loss_dev, hipStream_t stream);
+/* Batched forward of nrec device records into yhat (device). */
+int rks_mlp_forward(rks_mlp* net, const float* rec, int nrec, float* yhat, hipStream_t stream);
+/* Gradient of the mean loss over nrec records (a multiple of 64) without an
+ * update: grad (device, RKS_PROPOSER_NPARAM); yhat and loss_dev optional. */
+int rks_mlp_gradient(rks_mlp* net, const float* rec, int nrec, int loss_kind, double tau, float* grad, float* yhat,
+                     float* loss_dev, hipStream_t stream);
+/* Batch-1 forward of each record by one workgroup of `threads` (64..512):
+ * the code path the stepping kernels run after an accepted step. */
+int rks_mlp_infer_single(rks_mlp* net, const float* rec, int nrec, int threads, float* yhat, hipStream_t stream);
+/* Attach (net != NULL) or detach the proposer for the plan's evolves.  The plan
+ * must be adaptive, 1-D built-in (n >= 256 rfft, n >= 128 fft).  The net must
+ * outlive the plan's use of it. */
+int rks_plan_set_proposer(rks_plan* p, const rks_mlp* net, double margin);
+/* Record up to rec_cap training records per instance during evolve (0 turns off). */
+int rks_set_records(rks_plan* p, int rec_cap);
+/* Synchronise; counts[i] (host, optional) = records of instance i; dst_dev
+ * (optional, device) receives ninst x rec_cap x RKS_PROPOSER_REC floats. */
+int rks_read_records(rks_plan* p, float* dst_dev, int* counts, hipStream_t stream);
+
 /* Method tableau (host-only, no device needed): fills `bytes` >= rks_sizeof(0). */
 int rks_get_tableau(int method, void* out, int bytes);
 int rks_sizeof(int what);

@@ -20,8 +20,14 @@ EXPORTS = (
     "rks_plan_create", "rks_plan_destroy", "rks_plan_info", "rks_set_storage", "rks_evolve",
     "rks_resume", "rks_step", "rks_reset", "rks_nl_eval", "rks_coeffs", "rks_combine",
     "rks_error_s", "rks_get_stats", "rks_read_log", "rks_read_snapshots", "rks_get_tableau",
-    "rks_sizeof", "rks_last_error",
+    "rks_sizeof", "rks_last_error", "rks_set_config",
+    "rks_mlp_create", "rks_mlp_destroy", "rks_mlp_get_params", "rks_mlp_train", "rks_mlp_forward",
+    "rks_mlp_gradient", "rks_mlp_infer_single", "rks_plan_set_proposer", "rks_set_records",
+    "rks_read_records",
 )
+
+# learned step-size proposer (include/rkstiff_amd.h)
+PROPOSER_NIN, PROPOSER_REC, PROPOSER_NPARAM = 66, 68, 156673
 
 
 class Config(ctypes.Structure):
@@ -123,6 +129,18 @@ def load(path=None):
         lib.rks_read_snapshots.argtypes = [vp, ip, vp, P(dp), ip, vp]
         lib.rks_get_tableau.argtypes = [ip, vp, ip]
         lib.rks_sizeof.argtypes = [ip]
+        lib.rks_set_config.argtypes = [vp, P(Config), vp]
+        fp = P(ctypes.c_float)
+        lib.rks_mlp_create.argtypes = [fp, fp, fp, P(vp)]
+        lib.rks_mlp_destroy.argtypes = [vp]
+        lib.rks_mlp_get_params.argtypes = [vp, fp]
+        lib.rks_mlp_train.argtypes = [vp, vp, ip, vp, ip, ip, ip, dp, dp, vp, vp]
+        lib.rks_mlp_forward.argtypes = [vp, vp, ip, vp, vp]
+        lib.rks_mlp_gradient.argtypes = [vp, vp, ip, ip, dp, vp, vp, vp, vp]
+        lib.rks_mlp_infer_single.argtypes = [vp, vp, ip, ip, vp, vp]
+        lib.rks_plan_set_proposer.argtypes = [vp, vp, dp]
+        lib.rks_set_records.argtypes = [vp, ip]
+        lib.rks_read_records.argtypes = [vp, vp, P(ip), vp]
         lib.rks_last_error.restype = ctypes.c_char_p
         for name in EXPORTS:
             if name not in ("rks_last_error",):

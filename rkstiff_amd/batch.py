@@ -90,6 +90,20 @@ class BatchSolver:
             self._stats = self.plan.stats()
         return uf
 
+    # -- learned step-size proposer (rkstiff_amd.learned) ----------------------
+    def record(self, cap):
+        """Keep up to ``cap`` training records per trajectory during the next evolves."""
+        self.plan.set_records(cap)
+
+    def records(self):
+        """(records (ninst, cap, 68) device tensor, records per trajectory)."""
+        return self.plan.read_records()
+
+    def set_proposer(self, net, margin=0.0):
+        """Propose the step after each accept with ``net`` (a DtProposer); None restores the
+        reference's controller."""
+        self.plan.set_proposer(net, margin)
+
     def stats(self):
         """Per-trajectory counters: tc, steps, attempts, rejects, status, ..."""
         if self._stats is None:

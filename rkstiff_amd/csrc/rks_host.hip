@@ -160,6 +160,8 @@ extern template __global__ void rks_kernel<double, float, 13, NLK_CUBIC>(rks_lau
 #include "rkstiff_amd.h"
 
 int rks_build_tableau(int method, rks_tableau* out);
+rks_mlp_dev rks_mlp_device(const rks_mlp* n, double margin);
+int rks_mlp_device_of(const rks_mlp* n);
 
 namespace {
 
@@ -169,6 +171,13 @@ int fail(int code, const std::string& msg) {
   g_err = msg;
   return code;
 }
+
+}  // namespace
+
+// error channel shared with the proposer unit (rks_mlp.hip)
+void rks_set_last_error(const std::string& msg) { g_err = msg; }
+
+namespace {
 
 #define HIPCHK(x)                                                                          \
   do {                                                                                     \
@@ -258,6 +267,11 @@ struct rks_plan {
   // 2-D rfft2 engine (RKS_NL_AC2D_RFFT2): one grid size for the whole plan
   rks2_ctx* c2 = nullptr;
   int log2n2 = 0;
+  // learned step-size proposer (attached by rks_plan_set_proposer) and
+  // the training-record buffer (rks_set_records)
+  rks_mlp_dev mlp{};
+  float* d_rec = nullptr;
+  int rec_cap = 0;
 };
 
 static void fill_ctrl(rks_ctrl* c, const rks_config* cfg);
@@ -436,6 +450,7 @@ int rks_plan_create(int method, int nl_kind, int precision, const rks_config* cf
     r.ws = (char*)p->d_ws + p->ws_off[i];
     r.h_coeff = std::nan("");
     r.ubuf = -1;
+    r.rec_pending = -1;
     for (int s = 0; s < RKS_MAX_SLOTS; ++s) r.slot_map[s] = s;
   }
   HIPCHK(hipMemcpy(p->d_inst, p->h_inst.data(), sizeof(rks_inst) * ninst, hipMemcpyHostToDevice));
@@ -531,6 +546,7 @@ int rks_plan_destroy(rks_plan* p) {
   (void)hipFree(p->d_snap_t);
   (void)hipFree(p->d_queue);
   (void)hipFree(p->d_order);
+  (void)hipFree(p->d_rec);
   for (int l = 0; l < 14; ++l) (void)hipFree(p->d_tw[l]);
   rks2_destroy(p->c2);
   delete p;
@@ -598,7 +614,8 @@ static int launch_all(rks_plan* p, int mode, double h_mode, hipStream_t stream) 
     if (e != hipSuccess) return fail(RKS_ERR_HIP, std::string("2-D engine: ") + hipGetErrorString(e));
     return RKS_OK;
   }
-  rks_launch L;
+  rks_launch L{};
+  L.mlp = p->mlp;
   L.tab = p->d_tab;
   L.ctl = p->d_ctl;
   L.inst = p->d_inst;
@@ -687,6 +704,7 @@ int rks_reset(rks_plan* p, hipStream_t stream) {
     r.accept = 0;
     r.h_coeff = std::nan("");
     r.ubuf = -1;
+    r.rec_pending = -1;
     r.status = RKS_ST_OK;
     for (int s = 0; s < RKS_MAX_SLOTS; ++s) r.slot_map[s] = s;
   }
@@ -802,6 +820,72 @@ int rks_get_stats(rks_plan* p, rks_stats* out, hipStream_t stream) {
     s.snap_count = r.snap_count;
     s.nl_calls = r.nl_calls;
     s.coef_updates = r.coef_updates;
+  }
+  return RKS_OK;
+}
+
+// the proposer and its records need an adaptive 1-D plan whose workgroups
+// are whole waves (n >= 256 for rfft grids, n >= 128 for fft grids)
+static int learned_supported(const rks_plan* p) {
+  if (p->c2 || (p->nlk != RKS_NL_ADVECT_RFFT && p->nlk != RKS_NL_CUBIC_FFT))
+    return fail(RKS_ERR_INVALID, "the step-size proposer needs a 1-D built-in nonlinearity");
+  if (!p->tab.adaptive) return fail(RKS_ERR_INVALID, "the step-size proposer needs an adaptive method");
+  for (int l : p->bucket_log2)
+    if (rks_threads(l, p->nlk) < 64)
+      return fail(RKS_ERR_INVALID, "the step-size proposer needs n >= 256 (rfft) or n >= 128 (fft)");
+  return RKS_OK;
+}
+
+int rks_plan_set_proposer(rks_plan* p, const rks_mlp* net, double margin) {
+  if (!p) return fail(RKS_ERR_INVALID, "null plan");
+  if (!net) {
+    p->mlp = rks_mlp_dev{};
+    return RKS_OK;
+  }
+  int rc = learned_supported(p);
+  if (rc) return rc;
+  if (rks_mlp_device_of(net) != p->device) return fail(RKS_ERR_INVALID, "proposer and plan are on different devices");
+  if (!std::isfinite(margin)) return fail(RKS_ERR_INVALID, "margin must be finite");
+  p->mlp = rks_mlp_device(net, margin);
+  return RKS_OK;
+}
+
+int rks_set_records(rks_plan* p, int rec_cap) {
+  if (!p) return fail(RKS_ERR_INVALID, "null plan");
+  if (rec_cap < 0) return fail(RKS_ERR_INVALID, "negative record capacity");
+  if (rec_cap > 0) {
+    int rc = learned_supported(p);
+    if (rc) return rc;
+  }
+  HIPCHK(hipDeviceSynchronize());
+  (void)hipFree(p->d_rec);
+  p->d_rec = nullptr;
+  if (rec_cap > 0)
+    HIPCHK(hipMalloc(&p->d_rec, sizeof(float) * RKS_MLP_REC * (size_t)rec_cap * p->ninst));
+  HIPCHK(hipMemcpy(p->h_inst.data(), p->d_inst, sizeof(rks_inst) * p->ninst, hipMemcpyDeviceToHost));
+  for (int i = 0; i < p->ninst; ++i) {
+    rks_inst& r = p->h_inst[i];
+    r.rec = rec_cap > 0 ? p->d_rec + (size_t)RKS_MLP_REC * rec_cap * i : nullptr;
+    r.rec_cap = rec_cap;
+    r.rec_count = 0;
+    r.rec_pending = -1;
+  }
+  HIPCHK(hipMemcpy(p->d_inst, p->h_inst.data(), sizeof(rks_inst) * p->ninst, hipMemcpyHostToDevice));
+  p->rec_cap = rec_cap;
+  return RKS_OK;
+}
+
+int rks_read_records(rks_plan* p, float* dst_dev, int* counts, hipStream_t stream) {
+  if (!p) return fail(RKS_ERR_INVALID, "null plan");
+  if (!p->d_rec) return fail(RKS_ERR_INVALID, "records not enabled (rks_set_records)");
+  HIPCHK(hipStreamSynchronize(stream));
+  HIPCHK(hipMemcpy(p->h_inst.data(), p->d_inst, sizeof(rks_inst) * p->ninst, hipMemcpyDeviceToHost));
+  if (counts)
+    for (int i = 0; i < p->ninst; ++i) counts[i] = p->h_inst[i].rec_count;
+  if (dst_dev) {
+    HIPCHK(hipMemcpyAsync(dst_dev, p->d_rec, sizeof(float) * RKS_MLP_REC * (size_t)p->rec_cap * p->ninst,
+                          hipMemcpyDeviceToDevice, stream));
+    HIPCHK(hipStreamSynchronize(stream));
   }
   return RKS_OK;
 }

@@ -22,6 +22,7 @@
 
 #include "rks_fft.h"
 #include "rks_math.h"
+#include "rks_mlp.h"
 #include "rks_types.h"
 
 #define NLK_ADVECT 0
@@ -41,6 +42,7 @@ struct rks_launch {
   int* queue;
   const void* tw;  // exp(-2 pi i x / N), x < N, engine precision
   double h_mode;   // RKS_MODE_COEFFS
+  rks_mlp_dev mlp; // learned step-size proposer (mlp.on = 0: the reference's controller only)
 };
 
 // threads per workgroup for an N-point problem (N/16 elements... capped at 512,
@@ -641,11 +643,82 @@ __device__ __forceinline__ void eval_nl(GetK getk, cx<R>* buf, const cx<R>* twl,
 }
 
 // ---------------------------------------------------------------------------
+// learned step-size proposer: features of an accepted state, the training
+// record, and the proposal that replaces the controller's next step
+// ---------------------------------------------------------------------------
+// Inputs (raw, before standardization): for b < 64,
+//   f_b = 0.5 ln(mean |u_hat|^2 over bin b / n^2 + 1e-30),
+// where mode m has wavenumber index w = m (rfft layout) or min(m, n - m)
+// (fft layout) and lies in bin (64 w) / (n/2 + 1); f_64 = ln h of the accepted
+// step, f_65 = ln min(s, 1e4) of its attempt.  |u_hat| comes from the LDS
+// buffer where the fused final stage left it, else from the candidate array.
+// Returns the next step: the proposal h exp(clamp(y - margin)) when a
+// proposer is attached and its output is finite, else h_suggest.
+template <typename R, int LOG2N, int T, int NLK>
+__device__ __attribute__((noinline)) double learned_after_accept(const rks_launch& P, const rks_ctrl& ct, rks_inst* in,
+                                       const cx<R>* ucand, const cx<R>* buf, bool fused, double h_used,
+                                       double s_acc, double h_suggest, float* bins, float* scr,
+                                       int& rec_count, int& rec_pending) {
+  static_assert(T >= RKS_MLP_NBIN && T % 64 == 0, "one thread per spectral input, whole waves");
+  constexpr int N = 1 << LOG2N;
+  constexpr int M = (NLK == NLK_ADVECT) ? N / 2 + 1 : N;
+  const int t = threadIdx.x;
+  for (int j = t; j < 2 * RKS_MLP_NBIN; j += T) bins[j] = 0.f;
+  __syncthreads();
+  for (int m = t; m < M; m += T) {
+    double pw;
+    if (fused) {
+      const double a = (double)buf[m].re;
+      pw = a * a;
+    } else {
+      const cx<R> v = ucand[m];
+      pw = (double)v.re * (double)v.re + (double)v.im * (double)v.im;
+    }
+    const int w = (NLK == NLK_ADVECT) ? m : (m < N - m ? m : N - m);
+    const int b = (w * RKS_MLP_NBIN) / (N / 2 + 1);
+    atomicAdd(&bins[b], (float)pw);
+    atomicAdd(&bins[RKS_MLP_NBIN + b], 1.0f);
+  }
+  __syncthreads();
+  float f = 0.f;
+  if (t < RKS_MLP_NBIN)
+    f = 0.5f * logf(bins[t] / bins[RKS_MLP_NBIN + t] / ((float)N * (float)N) + 1e-30f);
+  __syncthreads();
+  if (t < RKS_MLP_NBIN) bins[t] = f;
+  if (t == 0) {
+    bins[RKS_MLP_NBIN] = (float)log(h_used);
+    bins[RKS_MLP_NBIN + 1] = (float)log(s_acc < RKS_MLP_S_CAP ? s_acc : RKS_MLP_S_CAP);
+  }
+  __syncthreads();
+  if (in->rec != nullptr && rec_count < in->rec_cap) {
+    float* r = in->rec + (size_t)rec_count * RKS_MLP_REC;
+    for (int j = t; j < RKS_MLP_NIN; j += T) r[j] = bins[j];
+    if (t == 0) {
+      r[RKS_MLP_NIN] = __builtin_nanf("");
+      r[RKS_MLP_NIN + 1] = 0.f;
+    }
+    rec_pending = rec_count;
+    ++rec_count;
+  }
+  if (P.mlp.on) {
+    rks_mlp_standardize<T>(P.mlp, bins, scr);
+    const float y = rks_mlp_infer<T>(P.mlp, scr, scr + RKS_MLP_K0);
+    if (!(isinf(y) || isnan(y))) {
+      double yl = (double)y - (double)P.mlp.margin;
+      const double lo = log(ct.min_s), hi = log(ct.max_s);
+      yl = yl < lo ? lo : (yl > hi ? hi : yl);
+      h_suggest = h_used * exp(yl);
+    }
+  }
+  return h_suggest;
+}
+
+// ---------------------------------------------------------------------------
 // one instance: evolve / step / nl / coefficients
 // ---------------------------------------------------------------------------
 template <typename R, typename RN, int LOG2N, int T, int NLK>
 __device__ void run_instance(const rks_launch& P, int idx, cx<R>* buf, const cx<R>* twl, double* red,
-                             int* lds_count) {
+                             int* lds_count, float* bins, float* scr) {
   using C = cx<R>;
   using CN = cx<RN>;
   constexpr int N = 1 << LOG2N;
@@ -698,6 +771,8 @@ __device__ void run_instance(const rks_launch& P, int idx, cx<R>* buf, const cx<
   int rejects = in->rejects, status = in->status, ubuf = in->ubuf;
   int log_count = in->log_count, snap_count = in->snap_count;
   int nl_calls = in->nl_calls, coef_updates = in->coef_updates;
+  int rec_count = in->rec_count, rec_pending = in->rec_pending;
+  double h_prop = in->h_prop;
   // slot map packed 4 bits per slot (no dynamically indexed register array)
   unsigned smap = 0;
 #pragma unroll
@@ -719,9 +794,12 @@ __device__ void run_instance(const rks_launch& P, int idx, cx<R>* buf, const cx<
     steps = attempts = rejects = 0;
     log_count = snap_count = 0;
     nl_calls = coef_updates = 0;
+    rec_count = 0;
+    rec_pending = -1;
     ubuf = -1;
     status = RKS_ST_RUNNING;
     if (tb.adaptive && tc + h > tf) h = tf - tc;  // solveras.py:620-621
+    h_prop = h;
     if (store_freq > 0) {
       for (int i = 0; i < MPT; ++i) {
         const int m = t + i * T;
@@ -757,7 +835,7 @@ __device__ void run_instance(const rks_launch& P, int idx, cx<R>* buf, const cx<
     const C* u = evolve ? cur_u() : uin;
     C* cand = evolve ? (ubuf == 0 ? cand1 : cand0) : (C*)io.u_out;
     int numloops = 0;
-    double h_suggest = h;
+    double h_suggest = h, s_acc = 0.0;
     // ---------------- attempt loop (solveras.py:384-410) ----------------
     for (;;) {
       if (h != h_coeff) {
@@ -887,6 +965,16 @@ __device__ void run_instance(const rks_launch& P, int idx, cx<R>* buf, const cx<
         in->log[2 * log_count + 1] = sv;
       }
       ++log_count;
+      if (evolve && rec_pending >= 0 && in->rec != nullptr) {
+        // target of the record left by the last accept: ln(h s / h_accepted)
+        if (t == 0) {
+          float* r = in->rec + (size_t)rec_pending * RKS_MLP_REC;
+          const bool ok = (h == h_prop) && !(isinf(sv) || isnan(sv)) && sv < 1e300;
+          r[RKS_MLP_NIN] = (float)log(h * sv / h_used);
+          r[RKS_MLP_NIN + 1] = ok ? 1.f : 0.f;
+        }
+        rec_pending = -1;
+      }
       if (isinf(sv) || isnan(sv) || sv < 1.0) {
         // reject (solveras.py:490-506)
         accept = 0;
@@ -913,6 +1001,7 @@ __device__ void run_instance(const rks_launch& P, int idx, cx<R>* buf, const cx<
       }
       // accept (solveras.py:543-554)
       accept = 1;
+      s_acc = sv;
       const double sc = sv < ct.max_s ? sv : ct.max_s;
       h_suggest = (sc > ct.incr_f) ? sc * h : h;
       break;
@@ -936,6 +1025,14 @@ __device__ void run_instance(const rks_launch& P, int idx, cx<R>* buf, const cx<
       status = RKS_ST_MINSTEP;
       break;
     }
+    if constexpr (T >= 64) {
+      if (tb.adaptive && tc < tf && (P.mlp.on || in->rec != nullptr)) {
+        h_suggest = learned_after_accept<R, LOG2N, T, NLK>(P, ct, in, cur_u(), buf, fused, h_used, s_acc,
+                                                           h_suggest, bins, scr, rec_count, rec_pending);
+        h_next = h_suggest;
+      }
+    }
+    h_prop = h_suggest;
     if (tb.adaptive) h = (tc + h_suggest > tf) ? tf - tc : h_suggest;  // solveras.py:638-641
     if (store_freq > 0 && (steps % store_freq) == 0) {
       C* dst = (C*)in->snap + (size_t)snap_count * M;
@@ -975,6 +1072,9 @@ __device__ void run_instance(const rks_launch& P, int idx, cx<R>* buf, const cx<
     in->snap_count = snap_count;
     in->nl_calls = nl_calls;
     in->coef_updates = coef_updates;
+    in->rec_count = rec_count;
+    in->rec_pending = rec_pending;
+    in->h_prop = h_prop;
 #pragma unroll
     for (int s = 0; s < RKS_MAX_SLOTS; ++s) in->slot_map[s] = (int)((smap >> (4 * s)) & 15u);
   }
@@ -992,8 +1092,15 @@ __global__ __launch_bounds__(rks_threads(LOG2N, NLK), rks_min_waves(LOG2N)) void
   __shared__ double red[2 * (T / 64 + 1)];
   __shared__ int s_item;
   __shared__ int s_count;
+  // proposer scratch: spectrum bins, and input/activation vectors (in the
+  // FFT buffer, idle between attempts, when it is large enough)
+  constexpr bool kScrInBuf =
+      rksfft::padded_size(1 << LOG2N) * sizeof(cx<R>) >= RKS_MLP_LDS_FLOATS * sizeof(float);
+  __shared__ float s_bins[2 * RKS_MLP_NBIN];
+  __shared__ float s_scr[kScrInBuf ? 1 : RKS_MLP_LDS_FLOATS];
   cx<R>* buf = (cx<R>*)rks_smem;
   cx<R>* twl = buf + rksfft::padded_size(1 << LOG2N);
+  float* scr = kScrInBuf ? (float*)rks_smem : s_scr;
   rksfft::load_twiddles<R, LOG2N, T>(twl, (const cx<R>*)P.tw);
   for (;;) {
     if (threadIdx.x == 0) s_item = atomicAdd(P.queue, 1);
@@ -1001,7 +1108,7 @@ __global__ __launch_bounds__(rks_threads(LOG2N, NLK), rks_min_waves(LOG2N)) void
     const int item = s_item;
     __syncthreads();
     if (item >= P.count) break;
-    run_instance<R, RN, LOG2N, T, NLK>(P, P.order[item], buf, twl, red, &s_count);
+    run_instance<R, RN, LOG2N, T, NLK>(P, P.order[item], buf, twl, red, &s_count, s_bins, scr);
     __syncthreads();
   }
 }

@@ -1,0 +1,173 @@
+// Learned step-size proposer: device-side pieces shared by the stepping
+// kernels (features of an accepted state, batch-1 inference inside the
+// trajectory's workgroup) and the training unit (rks_mlp.hip).
+//
+// Network: 66 inputs (padded to 96) -> 256 -> 256 -> 256 -> 1, exact GELU.
+// Weights are BF16, row-major [out][in]; the hidden layers multiply on the
+// matrix cores with v_mfma_f32_16x16x32_bf16 and FP32 accumulation.  The
+// output y estimates ln(h' s' / h), the log ratio between the largest step
+// that the next attempt would accept and the step just accepted; the
+// proposal is h * exp(clamp(y - margin, ln min_s, ln max_s)).
+//
+// The reference has no learned part (its controller, solveras.py:543-554,
+// keeps h unless s > incr_f); this is an opt-in addition.  The embedded
+// error estimate still judges every attempt and rejections use the
+// reference's rule.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define RKS_MLP_NBIN 64
+#define RKS_MLP_NIN 66
+#define RKS_MLP_K0 96
+#define RKS_MLP_HID 256
+#define RKS_MLP_REC 68        // floats per training record: 66 inputs, target, valid flag
+#define RKS_MLP_S_CAP 1.0e4   // ln s is taken of min(s, cap): s is DBL_MAX on a zero error
+// flat FP32 parameter vector: W1 [HID][K0], b1, W2 [HID][HID], b2, W3, b3, w4 [HID], b4
+#define RKS_MLP_OFF_W1 0
+#define RKS_MLP_OFF_B1 (RKS_MLP_OFF_W1 + RKS_MLP_HID * RKS_MLP_K0)
+#define RKS_MLP_OFF_W2 (RKS_MLP_OFF_B1 + RKS_MLP_HID)
+#define RKS_MLP_OFF_B2 (RKS_MLP_OFF_W2 + RKS_MLP_HID * RKS_MLP_HID)
+#define RKS_MLP_OFF_W3 (RKS_MLP_OFF_B2 + RKS_MLP_HID)
+#define RKS_MLP_OFF_B3 (RKS_MLP_OFF_W3 + RKS_MLP_HID * RKS_MLP_HID)
+#define RKS_MLP_OFF_W4 (RKS_MLP_OFF_B3 + RKS_MLP_HID)
+#define RKS_MLP_OFF_B4 (RKS_MLP_OFF_W4 + RKS_MLP_HID)
+#define RKS_MLP_NPARAM (RKS_MLP_OFF_B4 + 1)
+// BF16 copies of the four weight blocks, concatenated in the same order
+#define RKS_MLP_NWB (RKS_MLP_HID * RKS_MLP_K0 + 2 * RKS_MLP_HID * RKS_MLP_HID + RKS_MLP_HID)
+// LDS floats of one inference: padded input + two activation vectors
+#define RKS_MLP_LDS_FLOATS (RKS_MLP_K0 + 2 * RKS_MLP_HID)
+
+// What a stepping kernel needs of a trained proposer (all device pointers).
+struct rks_mlp_dev {
+  const uint16_t* wb;   // BF16 weights, RKS_MLP_NWB values (W1 | W2 | W3 | w4)
+  const float* param;   // FP32 parameters (biases are read from here)
+  const float* mu;      // input standardization: x = (f - mu) / sig, 66 each
+  const float* sig;
+  float margin;         // subtracted from the output before the clamp
+  int on;
+};
+
+typedef __bf16 rks_bf16x8 __attribute__((ext_vector_type(8)));
+typedef float rks_f32x4 __attribute__((ext_vector_type(4)));
+
+__host__ __device__ inline uint16_t rks_f2bf(float f) {
+  // round to nearest even (Inf and NaN stay Inf and NaN)
+  union {
+    float f;
+    uint32_t u;
+  } v;
+  v.f = f;
+  const uint32_t u = v.u + 0x7FFFu + ((v.u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+
+__host__ __device__ inline float rks_bf2f(uint16_t b) {
+  union {
+    float f;
+    uint32_t u;
+  } v;
+  v.u = (uint32_t)b << 16;
+  return v.f;
+}
+
+__device__ inline float rks_bfround(float f) { return rks_bf2f(rks_f2bf(f)); }
+
+__device__ inline float rks_gelu(float z) { return 0.5f * z * (1.0f + erff(z * 0.70710678118654752f)); }
+
+__device__ inline float rks_gelu_grad(float z) {
+  return 0.5f * (1.0f + erff(z * 0.70710678118654752f)) + z * __expf(-0.5f * z * z) * 0.39894228040143268f;
+}
+
+__device__ inline rks_bf16x8 rks_load_bf16x8(const uint16_t* p) {
+  // 16-byte aligned load of eight BF16 values
+  const uint4 v = *(const uint4*)p;
+  return __builtin_bit_cast(rks_bf16x8, v);
+}
+
+__device__ inline rks_bf16x8 rks_pack_bf16x8(const float* x) {
+  uint4 v;
+  v.x = (uint32_t)rks_f2bf(x[0]) | ((uint32_t)rks_f2bf(x[1]) << 16);
+  v.y = (uint32_t)rks_f2bf(x[2]) | ((uint32_t)rks_f2bf(x[3]) << 16);
+  v.z = (uint32_t)rks_f2bf(x[4]) | ((uint32_t)rks_f2bf(x[5]) << 16);
+  v.w = (uint32_t)rks_f2bf(x[6]) | ((uint32_t)rks_f2bf(x[7]) << 16);
+  return __builtin_bit_cast(rks_bf16x8, v);
+}
+
+// One hidden layer for a single input vector, all T threads of the workgroup.
+// out[r] = bf16(gelu(sum_k W[r][k] in[k] + b[r])), r < 256.  The vector is
+// column 0 of the 32x16 B operand (lanes 0, 16, 32, 48 carry it, the other
+// columns are zero); each 16-row tile of W is the A operand, so a wave
+// produces 16 outputs per K/32 instructions.  Column 0 of D sits in lanes
+// 0, 16, 32, 48 (rows 4 (lane/16) + j).
+template <int T, int K>
+__device__ inline void rks_mlp_layer1(const uint16_t* __restrict__ W, const float* __restrict__ b,
+                                      const float* in, float* out) {
+  static_assert(T % 64 == 0, "whole waves");
+  constexpr int NW = T / 64;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int col = lane & 15, kq = lane >> 4;
+  for (int ot = wave; ot < RKS_MLP_HID / 16; ot += NW) {
+    rks_f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+    const uint16_t* wrow = W + (size_t)(ot * 16 + col) * K + 8 * kq;
+#pragma unroll 4
+    for (int kt = 0; kt < K / 32; ++kt) {
+      const rks_bf16x8 a = rks_load_bf16x8(wrow + kt * 32);
+      rks_bf16x8 bv;
+      if (col == 0) {
+        bv = rks_pack_bf16x8(in + kt * 32 + 8 * kq);
+      } else {
+        const uint4 z = {0u, 0u, 0u, 0u};
+        bv = __builtin_bit_cast(rks_bf16x8, z);
+      }
+      acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bv, acc, 0, 0, 0);
+    }
+    if (col == 0) {
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int r = ot * 16 + 4 * kq + j;
+        out[r] = rks_bfround(rks_gelu(acc[j] + b[r]));
+      }
+    }
+  }
+  __syncthreads();
+}
+
+// Batch-1 inference by the whole workgroup.  x: LDS [K0] standardized
+// inputs; act: LDS [2 * HID].  Returns y (same value in every thread).
+template <int T>
+__device__ inline float rks_mlp_infer(const rks_mlp_dev& net, float* x, float* act) {
+  const uint16_t* w1 = net.wb;
+  const uint16_t* w2 = w1 + RKS_MLP_HID * RKS_MLP_K0;
+  const uint16_t* w3 = w2 + RKS_MLP_HID * RKS_MLP_HID;
+  const uint16_t* w4 = w3 + RKS_MLP_HID * RKS_MLP_HID;
+  float* a = act;
+  float* c = act + RKS_MLP_HID;
+  rks_mlp_layer1<T, RKS_MLP_K0>(w1, net.param + RKS_MLP_OFF_B1, x, a);
+  rks_mlp_layer1<T, RKS_MLP_HID>(w2, net.param + RKS_MLP_OFF_B2, a, c);
+  rks_mlp_layer1<T, RKS_MLP_HID>(w3, net.param + RKS_MLP_OFF_B3, c, a);
+  // head: one wave, FP32
+  float s = 0.f;
+  if (threadIdx.x < 64) {
+#pragma unroll
+    for (int j = 0; j < RKS_MLP_HID / 64; ++j) {
+      const int k = threadIdx.x + 64 * j;
+      s += rks_bf2f(w4[k]) * a[k];
+    }
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) s += __shfl_xor(s, o, 64);
+    if (threadIdx.x == 0) c[0] = s + net.param[RKS_MLP_OFF_B4];
+  }
+  __syncthreads();
+  const float y = c[0];
+  __syncthreads();
+  return y;
+}
+
+// Standardize raw inputs f[0..65] into x[0..95] (BF16-rounded, zero padded).
+template <int T>
+__device__ inline void rks_mlp_standardize(const rks_mlp_dev& net, const float* f, float* x) {
+  for (int k = threadIdx.x; k < RKS_MLP_K0; k += T)
+    x[k] = k < RKS_MLP_NIN ? rks_bfround((f[k] - net.mu[k]) / net.sig[k]) : 0.f;
+  __syncthreads();
+}

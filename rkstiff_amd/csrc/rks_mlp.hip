@@ -1,0 +1,586 @@
+// Training and batched inference of the learned step-size proposer.
+//
+// The network (rks_mlp.h) is trained on records that the stepping kernels
+// write during evolve: 66 inputs of an accepted state and the target
+// ln(h' s' / h) read from the next attempt.  Everything runs on the device:
+//   gather      records -> standardized BF16 inputs X [B][96], targets y
+//   mlp_gemm    C = A B on v_mfma_f32_16x16x32_bf16 (FP32 accumulate), 64x64
+//               tiles through LDS, with three epilogues: bias + GELU
+//               (forward), x GELU'(Z) (backward through a hidden layer), and
+//               plain FP32 (weight gradients)
+//   head        output layer, loss and its gradient, dZ of the last hidden layer
+//   colsum      bias gradients (deterministic column sums, no atomics)
+//   adam        Adam on the FP32 master parameters, refreshes the BF16 copies
+// One training step is 14 launches; nothing is allocated per step.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "rks_mlp.h"
+#include "rkstiff_amd.h"
+
+static_assert(RKS_PROPOSER_NIN == RKS_MLP_NIN && RKS_PROPOSER_REC == RKS_MLP_REC &&
+                  RKS_PROPOSER_NPARAM == RKS_MLP_NPARAM,
+              "public proposer sizes");
+
+void rks_set_last_error(const std::string& msg);  // rks_host.hip: the library's one error channel
+rks_mlp_dev rks_mlp_device(const rks_mlp* n, double margin);
+
+namespace {
+
+int mfail(int code, const std::string& msg);
+
+#define MLPCHK(x)                                                                                \
+  do {                                                                                           \
+    hipError_t e_ = (x);                                                                         \
+    if (e_ != hipSuccess) return mfail(RKS_ERR_HIP, std::string(#x) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+enum { EPI_F32 = 0, EPI_BIAS_GELU = 1, EPI_GELU_BWD = 2 };
+
+constexpr int GT = 64;        // output tile (rows and columns)
+constexpr int GK = 32;        // K step (one MFMA deep)
+constexpr int GLD = GK + 8;   // LDS row length in BF16 values (80 bytes, 16-byte aligned rows)
+
+struct GemmArgs {
+  int M, N, K;
+  const uint16_t* A;  // element (m, k) at A[m * sam + k * sak]
+  long sam, sak;
+  const uint16_t* B;  // element (k, n) at B[k * sbk + n * sbn]
+  long sbk, sbn;
+  float* C;           // EPI_F32: C[m * ldc + n]
+  int ldc;
+  const float* bias;  // EPI_BIAS_GELU: Z[m * ldo + n] = acc + bias[n], O = bf16(gelu(Z))
+  float* Z;
+  uint16_t* O;        // EPI_GELU_BWD: O[m * ldo + n] = bf16(acc * gelu'(Zin[m * ldo + n]))
+  const float* Zin;
+  int ldo;
+};
+
+// C [M x N] = A [M x K] B [K x N].  Requirements (checked by the host):
+// M % 64 == 0, K % 32 == 0, N % 16 == 0.  Four waves per 64x64 tile, each
+// owning a 32x32 quarter as 2x2 MFMA blocks of 16x16.
+template <int EPI>
+__global__ __launch_bounds__(256) void mlp_gemm(GemmArgs g) {
+  __shared__ __align__(16) uint16_t As[GT * GLD];  // [m][k]
+  __shared__ __align__(16) uint16_t Bs[GT * GLD];  // [n][k]
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const int m0 = blockIdx.y * GT, n0 = blockIdx.x * GT;
+  const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+  const int col = lane & 15, kq = lane >> 4;
+  rks_f32x4 acc[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) acc[i][j] = rks_f32x4{0.f, 0.f, 0.f, 0.f};
+  const bool a_kfast = (g.sak == 1);
+  const bool b_nfast = (g.sbn == 1);
+  for (int k0 = 0; k0 < g.K; k0 += GK) {
+#pragma unroll
+    for (int e = 0; e < GT * GK / 256; ++e) {
+      const int idx = t + 256 * e;
+      int mi, ki;
+      if (a_kfast) {
+        mi = idx / GK;
+        ki = idx % GK;
+      } else {
+        mi = idx % GT;
+        ki = idx / GT;
+      }
+      As[mi * GLD + ki] = g.A[(long)(m0 + mi) * g.sam + (long)(k0 + ki) * g.sak];
+      int ni;
+      if (b_nfast) {
+        ni = idx % GT;
+        ki = idx / GT;
+      } else {
+        ni = idx / GK;
+        ki = idx % GK;
+      }
+      const int n = n0 + ni;
+      Bs[ni * GLD + ki] = n < g.N ? g.B[(long)(k0 + ki) * g.sbk + (long)n * g.sbn] : (uint16_t)0;
+    }
+    __syncthreads();
+    rks_bf16x8 af[2], bf[2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i) af[i] = rks_load_bf16x8(&As[(wm + i * 16 + col) * GLD + 8 * kq]);
+#pragma unroll
+    for (int j = 0; j < 2; ++j) bf[j] = rks_load_bf16x8(&Bs[(wn + j * 16 + col) * GLD + 8 * kq]);
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i], bf[j], acc[i][j], 0, 0, 0);
+    __syncthreads();
+  }
+  // D layout: lane holds rows 4 * (lane / 16) + r of column lane % 16
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      const int n = n0 + wn + j * 16 + col;
+      if (n >= g.N) continue;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int m = m0 + wm + i * 16 + 4 * kq + r;
+        const float v = acc[i][j][r];
+        if constexpr (EPI == EPI_F32) {
+          g.C[(long)m * g.ldc + n] = v;
+        } else if constexpr (EPI == EPI_BIAS_GELU) {
+          const float z = v + g.bias[n];
+          g.Z[(long)m * g.ldo + n] = z;
+          g.O[(long)m * g.ldo + n] = rks_f2bf(rks_gelu(z));
+        } else {
+          g.O[(long)m * g.ldo + n] = rks_f2bf(v * rks_gelu_grad(g.Zin[(long)m * g.ldo + n]));
+        }
+      }
+    }
+}
+
+// Standardized BF16 inputs and targets of rows idx[b] (b < B; identity when
+// idx is null); rows b >= nvalid are zero (padding of a forward batch).
+__global__ void mlp_gather(const float* __restrict__ rec, const int* __restrict__ idx, int B, int nvalid,
+                           const float* __restrict__ mu, const float* __restrict__ sig, uint16_t* __restrict__ X,
+                           float* __restrict__ y) {
+  const int b = blockIdx.x;
+  if (b >= B) return;
+  const int k = threadIdx.x;
+  const bool valid = b < nvalid;
+  const long r = valid ? (idx ? idx[b] : b) : 0;
+  if (k < RKS_MLP_K0) {
+    float v = 0.f;
+    if (valid && k < RKS_MLP_NIN) v = (rec[r * RKS_MLP_REC + k] - mu[k]) / sig[k];
+    X[(long)b * RKS_MLP_K0 + k] = rks_f2bf(v);
+  }
+  if (k == 0) y[b] = valid ? rec[r * RKS_MLP_REC + RKS_MLP_NIN] : 0.f;
+}
+
+// Output layer, loss, d loss / d yhat and dZ of the last hidden layer; one
+// wave per sample.  loss_kind 0: pinball rho_tau(y - yhat); 1: 0.5 (yhat - y)^2.
+__global__ __launch_bounds__(256) void mlp_head(int B, int nvalid, const uint16_t* __restrict__ A3,
+                                                const float* __restrict__ Z3, const uint16_t* __restrict__ w4,
+                                                const float* __restrict__ param, const float* __restrict__ y,
+                                                int loss_kind, float tau, int backward, float* __restrict__ yhat,
+                                                float* __restrict__ gout, uint16_t* __restrict__ dZ3,
+                                                float* __restrict__ lossv) {
+  const int lane = threadIdx.x & 63;
+  const int b = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (b >= B) return;
+  float s = 0.f;
+#pragma unroll
+  for (int j = 0; j < RKS_MLP_HID / 64; ++j) {
+    const int k = lane + 64 * j;
+    s += rks_bf2f(w4[k]) * rks_bf2f(A3[(long)b * RKS_MLP_HID + k]);
+  }
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) s += __shfl_xor(s, o, 64);
+  const float yh = s + param[RKS_MLP_OFF_B4];
+  const bool valid = b < nvalid;
+  float gb = 0.f, L = 0.f;
+  if (valid) {
+    const float r = y[b] - yh;
+    if (loss_kind == 1) {
+      gb = (yh - y[b]) / (float)nvalid;
+      L = 0.5f * r * r;
+    } else {
+      gb = ((y[b] < yh ? 1.f : 0.f) - tau) / (float)nvalid;
+      L = r * (tau - (r < 0.f ? 1.f : 0.f));
+    }
+  }
+  if (lane == 0) {
+    if (valid) yhat[b] = yh;
+    gout[b] = gb;
+    lossv[b] = L;
+  }
+  if (backward) {
+#pragma unroll
+    for (int j = 0; j < RKS_MLP_HID / 64; ++j) {
+      const int k = lane + 64 * j;
+      dZ3[(long)b * RKS_MLP_HID + k] = rks_f2bf(gb * rks_bf2f(w4[k]) * rks_gelu_grad(Z3[(long)b * RKS_MLP_HID + k]));
+    }
+  }
+}
+
+// out[n] = sum_b D[b][n] (BF16 D), one thread per column, fixed order.
+__global__ void mlp_colsum(int B, int N, const uint16_t* __restrict__ D, float* __restrict__ out) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= N) return;
+  float s = 0.f;
+  for (int b = 0; b < B; ++b) s += rks_bf2f(D[(long)b * N + n]);
+  out[n] = s;
+}
+
+// Head gradients: gw4[k] = sum_b g_b A3[b][k]; gb4 = sum_b g_b; mean loss.
+__global__ void mlp_head_grad(int B, const float* __restrict__ g, const uint16_t* __restrict__ A3,
+                              const float* __restrict__ lossv, int nvalid, float* __restrict__ gw4,
+                              float* __restrict__ gb4, float* __restrict__ loss_out) {
+  const int k = threadIdx.x;
+  float s = 0.f;
+  for (int b = 0; b < B; ++b) s += g[b] * rks_bf2f(A3[(long)b * RKS_MLP_HID + k]);
+  gw4[k] = s;
+  if (k == 0) {
+    float sg = 0.f, sl = 0.f;
+    for (int b = 0; b < B; ++b) {
+      sg += g[b];
+      sl += lossv[b];
+    }
+    *gb4 = sg;
+    if (loss_out) *loss_out = sl / (float)nvalid;
+  }
+}
+
+// Index of flat parameter i in the BF16 weight copy (-1 for biases).
+__host__ __device__ inline int wb_index(int i) {
+  if (i < RKS_MLP_OFF_B1) return i;
+  if (i >= RKS_MLP_OFF_W2 && i < RKS_MLP_OFF_B2) return i - RKS_MLP_HID;
+  if (i >= RKS_MLP_OFF_W3 && i < RKS_MLP_OFF_B3) return i - 2 * RKS_MLP_HID;
+  if (i >= RKS_MLP_OFF_W4 && i < RKS_MLP_OFF_B4) return i - 3 * RKS_MLP_HID;
+  return -1;
+}
+
+__global__ void mlp_adam(const float* __restrict__ grad, float* __restrict__ param, float* __restrict__ m1,
+                         float* __restrict__ m2, uint16_t* __restrict__ wb, float lr, float b1, float b2,
+                         float eps, float bc1, float bc2) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= RKS_MLP_NPARAM) return;
+  const float gi = grad[i];
+  const float a = b1 * m1[i] + (1.f - b1) * gi;
+  const float v = b2 * m2[i] + (1.f - b2) * gi * gi;
+  m1[i] = a;
+  m2[i] = v;
+  const float p = param[i] - lr * (a / bc1) / (sqrtf(v / bc2) + eps);
+  param[i] = p;
+  const int w = wb_index(i);
+  if (w >= 0) wb[w] = rks_f2bf(p);
+}
+
+__global__ void mlp_to_bf16(const float* __restrict__ param, uint16_t* __restrict__ wb) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= RKS_MLP_NPARAM) return;
+  const int w = wb_index(i);
+  if (w >= 0) wb[w] = rks_f2bf(param[i]);
+}
+
+// Batch-1 inference of record r by one workgroup, through the same device
+// code the stepping kernels run after an accepted step.
+template <int T>
+__global__ __launch_bounds__(T) void mlp_infer_single(rks_mlp_dev net, const float* __restrict__ rec, int nrec,
+                                                       float* __restrict__ y) {
+  __shared__ float f[RKS_MLP_NIN];
+  __shared__ float scr[RKS_MLP_LDS_FLOATS];
+  const int r = blockIdx.x;
+  if (r >= nrec) return;
+  for (int k = threadIdx.x; k < RKS_MLP_NIN; k += T) f[k] = rec[(long)r * RKS_MLP_REC + k];
+  __syncthreads();
+  rks_mlp_standardize<T>(net, f, scr);
+  const float v = rks_mlp_infer<T>(net, scr, scr + RKS_MLP_K0);
+  if (threadIdx.x == 0) y[r] = v;
+}
+
+}  // namespace
+
+struct rks_mlp {
+  int device = 0;
+  float* d_param = nullptr;  // FP32 masters
+  float* d_m1 = nullptr;     // Adam moments
+  float* d_m2 = nullptr;
+  float* d_grad = nullptr;
+  uint16_t* d_wb = nullptr;  // BF16 weights
+  float* d_mu = nullptr;
+  float* d_sig = nullptr;
+  long adam_t = 0;
+  float margin = 0.f;
+  // activations for up to `cap` samples
+  int cap = 0;
+  uint16_t *X = nullptr, *A1 = nullptr, *A2 = nullptr, *A3 = nullptr, *D1 = nullptr, *D2 = nullptr, *D3 = nullptr;
+  float *Z1 = nullptr, *Z2 = nullptr, *Z3 = nullptr, *y = nullptr, *yh = nullptr, *g = nullptr, *lv = nullptr;
+  int* d_idx = nullptr;
+};
+
+namespace {
+
+int mfail(int code, const std::string& msg) {
+  rks_set_last_error(msg);
+  return code;
+}
+
+void free_ws(rks_mlp* n) {
+  void* ps[] = {n->X, n->A1, n->A2, n->A3, n->D1, n->D2, n->D3, n->Z1, n->Z2, n->Z3, n->y, n->yh, n->g, n->lv, n->d_idx};
+  for (void* p : ps) (void)hipFree(p);
+  n->X = n->A1 = n->A2 = n->A3 = n->D1 = n->D2 = n->D3 = nullptr;
+  n->Z1 = n->Z2 = n->Z3 = n->y = n->yh = n->g = n->lv = nullptr;
+  n->d_idx = nullptr;
+  n->cap = 0;
+}
+
+int ensure_ws(rks_mlp* n, int B) {
+  if (B <= n->cap) return RKS_OK;
+  free_ws(n);
+  const size_t hb = sizeof(uint16_t) * RKS_MLP_HID * (size_t)B, hf = sizeof(float) * RKS_MLP_HID * (size_t)B;
+  MLPCHK(hipMalloc(&n->X, sizeof(uint16_t) * RKS_MLP_K0 * (size_t)B));
+  MLPCHK(hipMalloc(&n->A1, hb));
+  MLPCHK(hipMalloc(&n->A2, hb));
+  MLPCHK(hipMalloc(&n->A3, hb));
+  MLPCHK(hipMalloc(&n->D1, hb));
+  MLPCHK(hipMalloc(&n->D2, hb));
+  MLPCHK(hipMalloc(&n->D3, hb));
+  MLPCHK(hipMalloc(&n->Z1, hf));
+  MLPCHK(hipMalloc(&n->Z2, hf));
+  MLPCHK(hipMalloc(&n->Z3, hf));
+  MLPCHK(hipMalloc(&n->y, sizeof(float) * B));
+  MLPCHK(hipMalloc(&n->yh, sizeof(float) * B));
+  MLPCHK(hipMalloc(&n->g, sizeof(float) * B));
+  MLPCHK(hipMalloc(&n->lv, sizeof(float) * B));
+  n->cap = B;
+  return RKS_OK;
+}
+
+template <int EPI>
+int gemm(const GemmArgs& a, hipStream_t s) {
+  if (a.M % GT || a.K % GK || a.N % 16) return mfail(RKS_ERR_INVALID, "gemm shape");
+  dim3 grid((a.N + GT - 1) / GT, a.M / GT);
+  hipLaunchKernelGGL(mlp_gemm<EPI>, grid, dim3(256), 0, s, a);
+  MLPCHK(hipGetLastError());
+  return RKS_OK;
+}
+
+GemmArgs fwd_args(int B, int K, const uint16_t* X, const uint16_t* W, const float* bias, float* Z, uint16_t* O) {
+  GemmArgs a{};
+  a.M = B;
+  a.N = RKS_MLP_HID;
+  a.K = K;
+  a.A = X;  // X [B][K]
+  a.sam = K;
+  a.sak = 1;
+  a.B = W;  // W^T: (k, n) = W[n][k]
+  a.sbk = 1;
+  a.sbn = K;
+  a.bias = bias;
+  a.Z = Z;
+  a.O = O;
+  a.ldo = RKS_MLP_HID;
+  return a;
+}
+
+// forward over B rows (B % 64 == 0) of gathered inputs; head, loss, dZ3 (when backward)
+int forward_batch(rks_mlp* n, int B, int nvalid, int loss_kind, float tau, int backward, hipStream_t s) {
+  const uint16_t* w1 = n->d_wb;
+  const uint16_t* w2 = w1 + RKS_MLP_HID * RKS_MLP_K0;
+  const uint16_t* w3 = w2 + RKS_MLP_HID * RKS_MLP_HID;
+  const uint16_t* w4 = w3 + RKS_MLP_HID * RKS_MLP_HID;
+  int rc;
+  if ((rc = gemm<EPI_BIAS_GELU>(fwd_args(B, RKS_MLP_K0, n->X, w1, n->d_param + RKS_MLP_OFF_B1, n->Z1, n->A1), s)))
+    return rc;
+  if ((rc = gemm<EPI_BIAS_GELU>(fwd_args(B, RKS_MLP_HID, n->A1, w2, n->d_param + RKS_MLP_OFF_B2, n->Z2, n->A2), s)))
+    return rc;
+  if ((rc = gemm<EPI_BIAS_GELU>(fwd_args(B, RKS_MLP_HID, n->A2, w3, n->d_param + RKS_MLP_OFF_B3, n->Z3, n->A3), s)))
+    return rc;
+  hipLaunchKernelGGL(mlp_head, dim3((B + 3) / 4), dim3(256), 0, s, B, nvalid, n->A3, n->Z3, w4, n->d_param, n->y,
+                     loss_kind, tau, backward, n->yh, n->g, n->D3, n->lv);
+  MLPCHK(hipGetLastError());
+  return RKS_OK;
+}
+
+// gradients of the mean loss into d_grad (after forward_batch with backward)
+int backward_batch(rks_mlp* n, int B, int nvalid, float* loss_out, hipStream_t s) {
+  const uint16_t* w1 = n->d_wb;
+  const uint16_t* w2 = w1 + RKS_MLP_HID * RKS_MLP_K0;
+  const uint16_t* w3 = w2 + RKS_MLP_HID * RKS_MLP_HID;
+  float* G = n->d_grad;
+  int rc;
+  hipLaunchKernelGGL(mlp_head_grad, dim3(1), dim3(RKS_MLP_HID), 0, s, B, n->g, n->A3, n->lv, nvalid,
+                     G + RKS_MLP_OFF_W4, G + RKS_MLP_OFF_B4, loss_out);
+  MLPCHK(hipGetLastError());
+  const uint16_t* acts[3] = {n->X, n->A1, n->A2};
+  const int kin[3] = {RKS_MLP_K0, RKS_MLP_HID, RKS_MLP_HID};
+  uint16_t* dz[3] = {n->D1, n->D2, n->D3};
+  const float* zs[3] = {n->Z1, n->Z2, n->Z3};
+  const uint16_t* ws[3] = {w1, w2, w3};
+  const int offW[3] = {RKS_MLP_OFF_W1, RKS_MLP_OFF_W2, RKS_MLP_OFF_W3};
+  const int offB[3] = {RKS_MLP_OFF_B1, RKS_MLP_OFF_B2, RKS_MLP_OFF_B3};
+  for (int l = 2; l >= 0; --l) {
+    // dW_l = dZ_l^T act_{l-1}: (m = out unit, k = sample), (k = sample, n = in unit)
+    GemmArgs a{};
+    a.M = RKS_MLP_HID;
+    a.N = kin[l];
+    a.K = B;
+    a.A = dz[l];
+    a.sam = 1;
+    a.sak = RKS_MLP_HID;
+    a.B = acts[l];
+    a.sbk = kin[l];
+    a.sbn = 1;
+    a.C = G + offW[l];
+    a.ldc = kin[l];
+    if ((rc = gemm<EPI_F32>(a, s))) return rc;
+    hipLaunchKernelGGL(mlp_colsum, dim3(RKS_MLP_HID / 64), dim3(64), 0, s, B, RKS_MLP_HID, dz[l], G + offB[l]);
+    MLPCHK(hipGetLastError());
+    if (l > 0) {
+      // dZ_{l-1} = (dZ_l W_l) * gelu'(Z_{l-1})
+      GemmArgs d{};
+      d.M = B;
+      d.N = RKS_MLP_HID;
+      d.K = RKS_MLP_HID;
+      d.A = dz[l];
+      d.sam = RKS_MLP_HID;
+      d.sak = 1;
+      d.B = ws[l];  // (k = out unit, n = in unit) = W[k][n]
+      d.sbk = RKS_MLP_HID;
+      d.sbn = 1;
+      d.Zin = zs[l - 1];
+      d.O = dz[l - 1];
+      d.ldo = RKS_MLP_HID;
+      if ((rc = gemm<EPI_GELU_BWD>(d, s))) return rc;
+    }
+  }
+  return RKS_OK;
+}
+
+int gather(rks_mlp* n, const float* rec, const int* idx, int B, int nvalid, hipStream_t s) {
+  hipLaunchKernelGGL(mlp_gather, dim3(B), dim3(128), 0, s, rec, idx, B, nvalid, n->d_mu, n->d_sig, n->X, n->y);
+  MLPCHK(hipGetLastError());
+  return RKS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int rks_mlp_create(const float* params, const float* mu, const float* sig, rks_mlp** out) {
+  if (!out || !params || !mu || !sig) return mfail(RKS_ERR_INVALID, "null argument");
+  *out = nullptr;
+  for (int k = 0; k < RKS_MLP_NIN; ++k)
+    if (!(sig[k] > 0.f) || !std::isfinite(mu[k]) || !std::isfinite(sig[k]))
+      return mfail(RKS_ERR_INVALID, "input scaling: sig must be > 0 and mu, sig finite");
+  for (int i = 0; i < RKS_MLP_NPARAM; ++i)
+    if (!std::isfinite(params[i])) return mfail(RKS_ERR_INVALID, "non-finite parameter");
+  rks_mlp* n = new rks_mlp();
+  auto bad = [&](hipError_t e) {
+    if (e == hipSuccess) return false;
+    rks_set_last_error(std::string("rks_mlp_create: ") + hipGetErrorString(e));
+    return true;
+  };
+  const size_t pb = sizeof(float) * RKS_MLP_NPARAM;
+  if (bad(hipGetDevice(&n->device)) || bad(hipMalloc(&n->d_param, pb)) || bad(hipMalloc(&n->d_m1, pb)) ||
+      bad(hipMalloc(&n->d_m2, pb)) || bad(hipMalloc(&n->d_grad, pb)) ||
+      bad(hipMalloc(&n->d_wb, sizeof(uint16_t) * RKS_MLP_NWB)) ||
+      bad(hipMalloc(&n->d_mu, sizeof(float) * RKS_MLP_NIN)) || bad(hipMalloc(&n->d_sig, sizeof(float) * RKS_MLP_NIN)) ||
+      bad(hipMemcpy(n->d_param, params, pb, hipMemcpyHostToDevice)) || bad(hipMemset(n->d_m1, 0, pb)) ||
+      bad(hipMemset(n->d_m2, 0, pb)) || bad(hipMemset(n->d_grad, 0, pb)) ||
+      bad(hipMemcpy(n->d_mu, mu, sizeof(float) * RKS_MLP_NIN, hipMemcpyHostToDevice)) ||
+      bad(hipMemcpy(n->d_sig, sig, sizeof(float) * RKS_MLP_NIN, hipMemcpyHostToDevice))) {
+    rks_mlp_destroy(n);
+    return RKS_ERR_HIP;
+  }
+  hipLaunchKernelGGL(mlp_to_bf16, dim3((RKS_MLP_NPARAM + 255) / 256), dim3(256), 0, 0, n->d_param, n->d_wb);
+  if (bad(hipGetLastError()) || bad(hipDeviceSynchronize())) {
+    rks_mlp_destroy(n);
+    return RKS_ERR_HIP;
+  }
+  *out = n;
+  return RKS_OK;
+}
+
+int rks_mlp_destroy(rks_mlp* n) {
+  if (!n) return RKS_OK;
+  (void)hipDeviceSynchronize();
+  free_ws(n);
+  void* ps[] = {n->d_param, n->d_m1, n->d_m2, n->d_grad, n->d_wb, n->d_mu, n->d_sig};
+  for (void* p : ps) (void)hipFree(p);
+  delete n;
+  return RKS_OK;
+}
+
+int rks_mlp_get_params(rks_mlp* n, float* params) {
+  if (!n || !params) return mfail(RKS_ERR_INVALID, "null argument");
+  MLPCHK(hipDeviceSynchronize());
+  MLPCHK(hipMemcpy(params, n->d_param, sizeof(float) * RKS_MLP_NPARAM, hipMemcpyDeviceToHost));
+  return RKS_OK;
+}
+
+int rks_mlp_train(rks_mlp* n, const float* rec, int nrec, const int* idx, int batch, int nsteps, int loss_kind,
+                  double tau, double lr, float* loss_dev, hipStream_t s) {
+  if (!n || !rec || !idx) return mfail(RKS_ERR_INVALID, "null argument");
+  if (batch <= 0 || batch % 64) return mfail(RKS_ERR_INVALID, "batch must be a positive multiple of 64");
+  if (nsteps < 0 || nrec <= 0) return mfail(RKS_ERR_INVALID, "nsteps >= 0 and nrec > 0 required");
+  if (loss_kind != 0 && loss_kind != 1) return mfail(RKS_ERR_INVALID, "loss_kind: 0 pinball, 1 mse");
+  if (!(tau > 0.0 && tau < 1.0) || !(lr > 0.0)) return mfail(RKS_ERR_INVALID, "tau in (0, 1) and lr > 0 required");
+  int rc = ensure_ws(n, batch);
+  if (rc) return rc;
+  const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+  for (int st = 0; st < nsteps; ++st) {
+    if ((rc = gather(n, rec, idx + (size_t)st * batch, batch, batch, s))) return rc;
+    if ((rc = forward_batch(n, batch, batch, loss_kind, (float)tau, 1, s))) return rc;
+    if ((rc = backward_batch(n, batch, batch, loss_dev ? loss_dev + st : nullptr, s))) return rc;
+    ++n->adam_t;
+    const float bc1 = (float)(1.0 - std::pow(0.9, (double)n->adam_t));
+    const float bc2 = (float)(1.0 - std::pow(0.999, (double)n->adam_t));
+    hipLaunchKernelGGL(mlp_adam, dim3((RKS_MLP_NPARAM + 255) / 256), dim3(256), 0, s, n->d_grad, n->d_param, n->d_m1,
+                       n->d_m2, n->d_wb, (float)lr, b1, b2, eps, bc1, bc2);
+    MLPCHK(hipGetLastError());
+  }
+  return RKS_OK;
+}
+
+int rks_mlp_forward(rks_mlp* n, const float* rec, int nrec, float* yhat, hipStream_t s) {
+  if (!n || !rec || !yhat || nrec < 0) return mfail(RKS_ERR_INVALID, "bad argument");
+  const int chunk = 4096;
+  int rc = ensure_ws(n, chunk);
+  if (rc) return rc;
+  for (int r0 = 0; r0 < nrec; r0 += chunk) {
+    const int nv = nrec - r0 < chunk ? nrec - r0 : chunk;
+    const int B = (nv + 63) / 64 * 64;
+    if ((rc = gather(n, rec + (size_t)r0 * RKS_MLP_REC, nullptr, B, nv, s))) return rc;
+    if ((rc = forward_batch(n, B, nv, 0, 0.5f, 0, s))) return rc;
+    MLPCHK(hipMemcpyAsync(yhat + r0, n->yh, sizeof(float) * nv, hipMemcpyDeviceToDevice, s));
+  }
+  return RKS_OK;
+}
+
+int rks_mlp_gradient(rks_mlp* n, const float* rec, int nrec, int loss_kind, double tau, float* grad, float* yhat,
+                     float* loss_dev, hipStream_t s) {
+  if (!n || !rec || !grad) return mfail(RKS_ERR_INVALID, "null argument");
+  if (nrec <= 0 || nrec % 64) return mfail(RKS_ERR_INVALID, "nrec must be a positive multiple of 64");
+  if (loss_kind != 0 && loss_kind != 1) return mfail(RKS_ERR_INVALID, "loss_kind: 0 pinball, 1 mse");
+  int rc = ensure_ws(n, nrec);
+  if (rc) return rc;
+  if ((rc = gather(n, rec, nullptr, nrec, nrec, s))) return rc;
+  if ((rc = forward_batch(n, nrec, nrec, loss_kind, (float)tau, 1, s))) return rc;
+  if ((rc = backward_batch(n, nrec, nrec, loss_dev, s))) return rc;
+  MLPCHK(hipMemcpyAsync(grad, n->d_grad, sizeof(float) * RKS_MLP_NPARAM, hipMemcpyDeviceToDevice, s));
+  if (yhat) MLPCHK(hipMemcpyAsync(yhat, n->yh, sizeof(float) * nrec, hipMemcpyDeviceToDevice, s));
+  return RKS_OK;
+}
+
+int rks_mlp_infer_single(rks_mlp* n, const float* rec, int nrec, int threads, float* yhat, hipStream_t s) {
+  if (!n || !rec || !yhat || nrec < 0) return mfail(RKS_ERR_INVALID, "bad argument");
+  if (nrec == 0) return RKS_OK;
+  rks_mlp_dev d = rks_mlp_device(n, 0.0);
+  switch (threads) {
+    case 64: hipLaunchKernelGGL(mlp_infer_single<64>, dim3(nrec), dim3(64), 0, s, d, rec, nrec, yhat); break;
+    case 128: hipLaunchKernelGGL(mlp_infer_single<128>, dim3(nrec), dim3(128), 0, s, d, rec, nrec, yhat); break;
+    case 256: hipLaunchKernelGGL(mlp_infer_single<256>, dim3(nrec), dim3(256), 0, s, d, rec, nrec, yhat); break;
+    case 512: hipLaunchKernelGGL(mlp_infer_single<512>, dim3(nrec), dim3(512), 0, s, d, rec, nrec, yhat); break;
+    default: return mfail(RKS_ERR_INVALID, "threads must be 64, 128, 256 or 512");
+  }
+  MLPCHK(hipGetLastError());
+  return RKS_OK;
+}
+
+}  // extern "C"
+
+// device view of a proposer for the stepping kernels (rks_host.hip)
+rks_mlp_dev rks_mlp_device(const rks_mlp* n, double margin) {
+  rks_mlp_dev d{};
+  if (!n) return d;
+  d.wb = n->d_wb;
+  d.param = n->d_param;
+  d.mu = n->d_mu;
+  d.sig = n->d_sig;
+  d.margin = (float)margin;
+  d.on = 1;
+  return d;
+}
+
+int rks_mlp_device_of(const rks_mlp* n) { return n ? n->device : -1; }

@@ -146,6 +146,13 @@ typedef struct rks_inst {
   int rejects, status, ubuf, numloops;  // ubuf: -1 = caller's u_in, 0/1 = candidate buffers
   int log_count, snap_count, nl_calls, coef_updates;
   int slot_map[RKS_MAX_SLOTS];
+  // learned step-size proposer: training records written during evolve
+  // (RKS_MLP_REC floats each: 66 inputs of the accepted state, target, valid)
+  float* rec;
+  int rec_cap, rec_count;
+  int rec_pending;     // record whose target the next attempt fills (-1: none)
+  int pad_rec;
+  double h_prop;       // step proposed after the last accept, before the tf clamp
 } rks_inst;
 
 // Per-launch inputs, written by the host before every launch.

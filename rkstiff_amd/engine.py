@@ -117,6 +117,24 @@ class Plan:
             _lib.check(self.lib.rks_set_storage(self.h, store_freq, snap_cap, log_cap))
             self.storage = (store_freq, snap_cap, log_cap)
 
+    # -- learned step-size proposer (rkstiff_amd.learned) ----------------------
+    def set_records(self, cap):
+        torch.cuda.current_stream().synchronize()
+        _lib.check(self.lib.rks_set_records(self.h, int(cap)))
+        self.rec_cap = int(cap)
+
+    def read_records(self):
+        """(records (ninst, cap, 68) float32 device tensor, records per instance)."""
+        cap = getattr(self, "rec_cap", 0)
+        counts = (ctypes.c_int * self.ninst)()
+        out = torch.empty((self.ninst, cap, _lib.PROPOSER_REC), dtype=torch.float32, device="cuda")
+        _lib.check(self.lib.rks_read_records(self.h, ctypes.c_void_p(out.data_ptr()), counts, _stream()))
+        return out, list(counts)
+
+    def set_proposer(self, net, margin=0.0):
+        _lib.check(self.lib.rks_plan_set_proposer(self.h, net.h if net is not None else None, float(margin)))
+        self.proposer = net  # the plan's kernels read the network's device buffers
+
     # -- launches --------------------------------------------------------------
     def _check_state(self, tensors):
         for i, t in enumerate(tensors):

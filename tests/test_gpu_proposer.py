@@ -1,0 +1,241 @@
+"""Learned step-size proposer on the device against the oracle (oracle/mlp.py).
+
+* network kernels: outputs and gradients of the MFMA forward/backward vs the
+  oracle at the device's BF16 rounding points; one Adam step vs Adam applied
+  to the device gradient; the batch-1 path of the stepping kernels vs the
+  batched GEMM path;
+* records written during evolve: inputs vs the oracle features of the stored
+  accepted states, targets vs the attempt log;
+* proposals inside the stepping kernel vs the batch-1 path on the records;
+* what a trained network gains over the reference's controller.
+"""
+import math
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import mlp as omlp
+from rkstiff_amd import _lib
+from rkstiff_amd.batch import BatchSolver
+from rkstiff_amd.config import SolverConfig
+from rkstiff_amd.engine import Plan
+from rkstiff_amd.learned import NIN, NPARAM, REC, DtProposer, init_params, training_set
+from rkstiff_amd.workloads import ks_ensemble, nls_ensemble
+
+pytestmark = pytest.mark.gpu
+
+
+def synth_records(N, seed):
+    rng = np.random.default_rng(seed)
+    r = np.zeros((N, REC), np.float32)
+    r[:, :64] = rng.normal(-10.0, 3.0, (N, 64))
+    r[:, 64] = rng.uniform(-8.0, -2.0, N)
+    r[:, 65] = rng.uniform(0.0, 1.5, N)
+    r[:, 66] = rng.normal(0.05, 0.1, N)
+    r[:, 67] = 1.0
+    return r
+
+
+def oracle_params(seed):
+    p = omlp.init_params(seed)
+    rng = np.random.default_rng(seed + 7)
+    for k in ("b1", "b2", "b3", "b4"):
+        p[k] = (0.1 * rng.standard_normal(p[k].shape)).astype(np.float32).astype(np.float64)
+    return p
+
+
+def make_net(p, rec):
+    mu = rec[:, :NIN].astype(np.float64).mean(axis=0).astype(np.float32)
+    sd = np.maximum(rec[:, :NIN].astype(np.float64).std(axis=0), 1e-3).astype(np.float32)
+    net = DtProposer(params=omlp.flatten(p).astype(np.float32), mu=mu, sigma=sd)
+    return net, mu, sd
+
+
+def oracle_inputs(rec, mu, sd):
+    # the device standardizes in FP32
+    x = (rec[:, :NIN] - mu) / sd
+    return omlp.standardize(x.astype(np.float64), np.zeros(NIN), np.ones(NIN))
+
+
+def test_forward_paths_agree_with_oracle():
+    rec = synth_records(256, 1)
+    p = oracle_params(2)
+    net, mu, sd = make_net(p, rec)
+    d = torch.from_numpy(rec).cuda()
+    yb = net.predict(d).cpu().numpy()
+    singles = [net.predict_single(d, threads=t).cpu().numpy() for t in (64, 128, 256, 512)]
+    for ys in singles[1:]:
+        np.testing.assert_array_equal(ys, singles[0])   # same MFMA sequence per output tile
+    yo, _ = omlp.forward(p, oracle_inputs(rec, mu, sd), quant=True)
+    print("max |batched - oracle|", np.max(np.abs(yb - yo)), "max |single - batched|",
+          np.max(np.abs(singles[0] - yb)))
+    assert np.max(np.abs(yb - yo)) < 2e-3
+    assert np.max(np.abs(singles[0] - yb)) < 2e-3
+
+
+@pytest.mark.parametrize("kind", ["mse", "pinball"])
+def test_gradient_matches_oracle(kind):
+    rec = synth_records(512, 3)
+    p = oracle_params(4)
+    net, mu, sd = make_net(p, rec)
+    X = oracle_inputs(rec, mu, sd)
+    yo, cache = omlp.forward(p, X, quant=True)
+    if kind == "pinball":
+        # residuals away from the kink: the loss gradient is then the same sign on both sides
+        rec[:, 66] = (yo + np.where(np.arange(512) % 3 == 0, 0.4, -0.4)).astype(np.float32)
+    y = rec[:, 66].astype(np.float64)
+    g = omlp.loss_grad(yo, y, kind, tau=0.2)
+    go = omlp.backward(p, cache, g, quant=True)
+    grad, yhat, lv = net.gradient(torch.from_numpy(rec).cuda(), loss=kind, tau=0.2)
+    grad = grad.cpu().numpy().astype(np.float64)
+    gd = omlp.unflatten(grad)
+    assert abs(lv - omlp.loss(yo, y, kind, tau=0.2)) < 1e-3 * (1 + abs(lv))
+    for name, _ in omlp.SHAPES:
+        a, b = gd[name], go[name]
+        rel = np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-12)
+        print(kind, name, "rel", rel)
+        assert rel < 5e-3, (name, rel)
+
+
+def test_adam_step_matches_gradient_update():
+    rec = synth_records(256, 5)
+    p = oracle_params(6)
+    net, mu, sd = make_net(p, rec)
+    d = torch.from_numpy(rec).cuda()
+    grad, _, _ = net.gradient(d, loss="mse")
+    v0 = net.params().astype(np.float64)
+    lr = 1e-3
+    net.train(d, steps=1, batch=256, lr=lr, loss="mse", seed=0)  # one full batch (a permutation)
+    v1 = net.params().astype(np.float64)
+    g = grad.cpu().numpy().astype(np.float64)
+    ref, _, _ = omlp.adam(v0, g, np.zeros(NPARAM), np.zeros(NPARAM), 1, lr)
+    err = np.abs(v1 - ref)
+    print("max |adam - ref|", err.max())
+    # the batch order differs (a permutation), so the FP32 sums differ in the last bits
+    assert np.mean(err < 1e-6 + 1e-5 * np.abs(v0 - ref)) > 0.999
+    assert err.max() < 2.0 * lr
+
+
+def test_training_reduces_loss():
+    rec = synth_records(4096, 7)
+    # a smooth target of the inputs
+    rec[:, 66] = (0.3 * np.tanh(rec[:, 65] - 0.7) - 0.05 * (rec[:, 64] + 5.0)
+                  + 0.02 * (rec[:, 3] + 10.0)).astype(np.float32)
+    d = torch.from_numpy(rec).cuda()
+    net = DtProposer.for_data(d, seed=0)
+    before = net.gradient(d[:4096], loss="mse")[2]
+    hist = net.train(d, steps=600, batch=512, lr=1e-3, loss="mse")
+    after = net.gradient(d[:4096], loss="mse")[2]
+    print("mse before", before, "after", after, "last steps", hist[-5:])
+    assert np.all(np.isfinite(hist))
+    assert after < 0.05 * before
+
+
+def _run_recorded(method, ens, prec, tf, cap, eps, proposer=None, margin=0.0):
+    cfg = SolverConfig(epsilon=eps)
+    descs = [nl.descriptor(torch.as_tensor(L)) for (_, _, L, nl, _) in ens]
+    plan = Plan(method, ens[0][3], {"f64": _lib.F64, "f32": _lib.F32}[prec], cfg, None, descs)
+    dt = plan.dtype
+    u0 = [torch.as_tensor(u).to(dt).cuda() for (*_, u) in ens]
+    uf = [torch.empty_like(u) for u in u0]
+    plan.set_storage(1, cap, 4 * cap)
+    plan.set_records(cap)
+    if proposer is not None:
+        plan.set_proposer(proposer, margin)
+    plan.evolve(u0, uf, [0.0] * len(ens), [tf] * len(ens), [tf / 100.0] * len(ens))
+    st = plan.stats()
+    recs, counts = plan.read_records()
+    return plan, st, recs.cpu().numpy(), counts
+
+
+def _accepted(log):
+    s = log[:, 1]
+    return np.where(np.isfinite(s) & (s >= 1.0))[0]
+
+
+@pytest.mark.parametrize("case", ["ks_if34_f64", "nls_etd35_f32"])
+def test_records_match_oracle_features(case):
+    if case.startswith("ks"):
+        ens = ks_ensemble(3, seed=11, sizes=(256, 512))
+        method, prec, tf, real = "IF34", "f64", 10.0, True
+    else:
+        ens = nls_ensemble(3, seed=12, sizes=(256, 1024))
+        method, prec, tf, real = "ETD35", "f32", 1.0, False
+    plan, st, recs, counts = _run_recorded(method, ens, prec, tf, 600, 1e-5)
+    for i, (n, *_rest) in enumerate(ens):
+        s = st[i]
+        assert s["status"] == _lib.RKS_OK and s["snap_count"] == s["steps"] + 1 <= 600
+        log = plan.attempt_log(i, s["log_count"])
+        acc = _accepted(log)
+        assert len(acc) == s["steps"]
+        times, snaps = plan.snapshots(i, s["snap_count"])
+        snaps = snaps.cpu().numpy()
+        assert counts[i] == s["steps"] - 1          # no record after the step that reaches tf
+        for k in range(counts[i]):
+            a = acc[k]
+            h, sv = log[a]
+            f = omlp.features(snaps[k + 1].astype(np.complex128), n, real, h, sv)
+            r = recs[i, k]
+            np.testing.assert_allclose(r[:64], f[:64], rtol=0, atol=2e-4)
+            assert r[64] == pytest.approx(f[64], rel=1e-6) and r[65] == pytest.approx(f[65], rel=1e-5, abs=1e-6)
+            hn, sn = log[a + 1]
+            assert r[66] == pytest.approx(math.log(hn * sn / h), rel=1e-5, abs=1e-6)
+            # invalid exactly when the next attempt was clamped to tf (or its error vanished)
+            clamped = hn == tf - times[k + 1]
+            assert (r[67] == 0.0) == (clamped or not sn < 1e300), (i, k, r[67], hn, tf - times[k + 1])
+
+
+def _ks_threads(n):
+    # workgroup size of the stepping kernel for an n-point rfft grid (rks_threads)
+    lo = min(n // 4, 64)
+    return min(max(n // 16, lo), 512)
+
+
+def test_in_kernel_proposals_match_single_path():
+    ens = ks_ensemble(4, seed=21, sizes=(256, 1024, 4096))
+    plan0, st0, recs0, counts0 = _run_recorded("IF34", ens, "f64", 10.0, 800, 1e-5)
+    data = training_set(torch.from_numpy(recs0).cuda(), counts0, plan0.cfg)
+    net = DtProposer.for_data(data, seed=1)
+    net.train(data, steps=100, batch=128, lr=1e-3, tau=0.2)
+    margin = 0.05
+    plan, st, recs, counts = _run_recorded("IF34", ens, "f64", 10.0, 800, 1e-5, proposer=net, margin=margin)
+    lo, hi = math.log(plan.cfg.min_s), math.log(plan.cfg.max_s)
+    checked = 0
+    for i, (n, *_r) in enumerate(ens):
+        s = st[i]
+        assert s["status"] == _lib.RKS_OK
+        log = plan.attempt_log(i, s["log_count"])
+        acc = _accepted(log)
+        times, _ = plan.snapshots(i, s["snap_count"])
+        y = net.predict_single(torch.from_numpy(np.ascontiguousarray(recs[i, :counts[i]])).cuda(),
+                               threads=_ks_threads(n)).cpu().numpy().astype(np.float64)
+        for k in range(counts[i]):
+            h = log[acc[k], 0]
+            if log[acc[k] + 1, 0] == 10.0 - times[k + 1]:
+                continue   # clamped to tf
+            prop = h * math.exp(min(max(float(y[k]) - float(np.float32(margin)), lo), hi))
+            assert log[acc[k] + 1, 0] == pytest.approx(prop, rel=1e-12), (i, k)
+            checked += 1
+    assert checked > 50
+
+
+def test_proposer_rejects_unsupported_plans():
+    net = DtProposer(seed=0)
+    kx = 2 * np.pi * np.fft.rfftfreq(128, d=1.0)
+    from rkstiff_amd.nonlinear import AdvectiveNL
+    small = BatchSolver("IF34", [(torch.as_tensor(kx**2 * (1 - kx**2)), AdvectiveNL(kx, -1.0))])
+    with pytest.raises(ValueError, match="n >= 256"):
+        small.set_proposer(net)
+    with pytest.raises(ValueError, match="n >= 256"):
+        small.record(10)
+    ens = ks_ensemble(1, seed=1, sizes=(256,))
+    const = BatchSolver("IF4", [(torch.as_tensor(ens[0][2]), ens[0][3])])
+    with pytest.raises(ValueError, match="adaptive"):
+        const.set_proposer(net)
+    with pytest.raises(ValueError):
+        DtProposer(params=np.zeros(10, np.float32))
+    bad = init_params(0)
+    bad[5] = np.nan
+    with pytest.raises(ValueError):
+        DtProposer(params=bad)
