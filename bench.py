@@ -24,6 +24,14 @@ Also measured and reported under "workloads" (one timed pass each):
   allen_cahn_2d 64 trajectories of u_t = 0.01 Lap u + u - u^3 on a 512 x 512
                 periodic grid (rfft2), u0 ~ U[-0.05, 0.05] low-passed to
                 |k| <= 32, ETD35, epsilon = 1e-5, tf = 10
+  learned_dt    the learned step-size proposer on a mixed batch: KS as above
+                (IF34, epsilon 1e-5, tf 100) and NLS with n ~ U{1024..8192} and
+                1-3 random solitons (ETD35, epsilon 1e-6, tf 20), --learned-count
+                of each, FP32.  The network is trained on the device on ~2e5
+                records from the solver's own runs (separate seeds); the line
+                reports the reference controller and the proposer side by side
+                (trajectories/s, attempts, rejections, coefficient
+                regenerations, error vs FP64 with the step-sequence floor)
 
 Reported beside the value:
   * the plain per-instance FP64 drop-in (one ETD35.evolve per trajectory),
@@ -35,7 +43,8 @@ Reported beside the value:
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--batch B]
                        [--modes f64,mixed,f32] [--plain P] [--no-cpu]
-                       [--workloads nls,ks,ac2d] [--ks-count C] [--ac-batch A]
+                       [--workloads nls,ks,ac2d,learned] [--ks-count C] [--ac-batch A]
+                       [--learned-count Q]
                        [--copy-mib X]   (first run a device copy of X MiB:
                                          the reference point for HBM counters)
 """
@@ -81,21 +90,9 @@ def nls_lin_op(n):
 
 
 def ks_inputs(count, seed=2024, sizes=(256, 512, 1024, 2048, 4096)):
-    """Per trajectory, in this order: n ~ U{sizes}, domain ~ U[22, 200], a_k ~ N(0, 0.1^2)
-    and theta_k ~ U[0, 2 pi) for k = 1..8; u0 = sum_k a_k cos(2 pi k x / domain + theta_k)."""
-    rng = np.random.default_rng(seed)
-    out = []
-    for _ in range(count):
-        n = int(rng.choice(sizes))
-        dom = float(rng.uniform(22.0, 200.0))
-        amp = rng.normal(0.0, 0.1, 8)
-        th = rng.uniform(0.0, 2 * np.pi, 8)
-        x = np.arange(n) * (dom / n)
-        kx = 2 * np.pi * np.fft.rfftfreq(n, d=dom / n)
-        k = np.arange(1, 9)
-        u0 = np.sum(amp[:, None] * np.cos(2 * np.pi * k[:, None] * x[None, :] / dom + th[:, None]), axis=0)
-        out.append((n, kx, kx**2 * (1 - kx**2), np.fft.rfft(u0)))
-    return out
+    """(n, kx, L, u0_hat) per trajectory of rkstiff_amd.workloads.ks_ensemble (seeded)."""
+    from rkstiff_amd.workloads import ks_ensemble
+    return [(n, kx, L, u0) for n, kx, L, _nl, u0 in ks_ensemble(count, seed, sizes)]
 
 
 def ac_inputs(count, n=512, seed=7, kcut=32.0):
@@ -281,6 +278,94 @@ def ac_workload(args):
             "unit": "trajectories/s", "launches": "hipGraph-replayed attempt sequence", "results": summarize(res)}
 
 
+def learned_workload(args):
+    """Reference controller vs the learned proposer on a mixed KS + NLS batch (FP32)."""
+    import torch
+    from rkstiff_amd.batch import BatchSolver
+    from rkstiff_amd.config import SolverConfig
+    from rkstiff_amd.learned import DtProposer, training_set
+    from rkstiff_amd.workloads import ks_ensemble, nls_ensemble
+
+    fams = {"ks": ("IF34", 1e-5, 100.0, ks_ensemble, 48), "nls": ("ETD35", 1e-6, 20.0, nls_ensemble, 96)}
+    POLICIES = {"propose": dict(policy="propose", margin=0.02, keep=(0.0, float(np.log(1.25)))),
+                "guard": dict(policy="guard", margin=0.0, keep=(0.0, 0.0))}
+
+    def solver(method, ens, prec, eps, safety=0.8):
+        return BatchSolver(method, [(torch.as_tensor(L), nl) for (_, _, L, nl, _) in ens], precision=prec,
+                           config=SolverConfig(epsilon=eps, safety_f=safety))
+
+    def run(s, ens, tf):
+        u0 = [torch.as_tensor(e[4]).to(s.dtype).cuda() for e in ens]
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        uf = s.evolve(u0, 0.0, tf)
+        torch.cuda.synchronize()
+        return uf, s.stats(), time.perf_counter() - t
+
+    def counts(st):
+        return {k: int(sum(x[k] for x in st)) for k in ("attempts", "rejects", "steps", "coef_updates")}
+
+    # training records: the solver's own accept/reject logs, reference controller
+    datas = []
+    for name, (method, eps, tf, gen, ntrain) in fams.items():
+        ens = gen(ntrain, 7000 + len(name))
+        s = solver(method, ens, "f32", eps)
+        s.record(6000)
+        run(s, ens, tf)
+        datas.append(training_set(*s.records(), s.plan.cfg))
+    data = torch.cat(datas)
+    g = torch.Generator().manual_seed(0)
+    data = data[torch.randperm(data.shape[0], generator=g)[: args.learned_records].to(data.device)].contiguous()
+    net = DtProposer.for_data(data, seed=0)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    hist = net.train(data, steps=args.learned_steps, batch=1024, lr=1e-3, loss="pinball", tau=0.2)
+    torch.cuda.synchronize()
+    t_train = time.perf_counter() - t
+    # forward + backward FLOPs of one step: 3 GEMM passes per hidden layer, 2 flops per MAC
+    macs = 96 * 256 + 2 * 256 * 256 + 256
+    train_tflops = 3 * 2 * 1024 * macs * args.learned_steps / t_train / 1e12
+
+    out = {"count_per_family": args.learned_count, "records": int(data.shape[0]),
+           "train": {"steps": args.learned_steps, "batch": 1024, "loss": "pinball tau 0.2",
+                     "loss_first": sig(float(hist[:10].mean())), "loss_last": sig(float(hist[-50:].mean())),
+                     "seconds": sig(t_train),
+                     "mfma": {"achieved": sig(train_tflops), "peak": 2500.0, "unit": "TFLOP/s",
+                              "frac": sig(train_tflops / 2500.0),
+                              "what": "BF16 GEMMs of training incl. launch gaps (wall)"}},
+           "policies": {k: {kk: (list(vv) if isinstance(vv, tuple) else vv) for kk, vv in v.items()}
+                        for k, v in POLICIES.items()}, "families": {}}
+    tot = {"std": 0.0}
+    for name, (method, eps, tf, gen, _) in fams.items():
+        ens = gen(args.learned_count, 9000 + len(name))
+        ref, _, _ = run(solver(method, ens, "f64", eps), ens, tf)
+        floor, _, _ = run(solver(method, ens, "f64", eps, safety=0.79), ens, tf)
+        s = solver(method, ens, "f32", eps)
+        u_std, st_std, t_std = run(s, ens, tf)
+        e_fl = rel_errors(floor, ref)
+
+        def entry(u, st, t):
+            e = rel_errors(u, ref)
+            return {"traj_per_s": sig(len(ens) / t), **counts(st), "rel_l2_vs_fp64_max": sig(float(e.max())),
+                    "rel_l2_vs_fp64_median": sig(float(np.median(e)))}
+
+        fam = {"method": method, "tf": tf, "epsilon": eps, "std": entry(u_std, st_std, t_std),
+               "fp64_step_sequence_floor": {"rel_l2_max": sig(float(e_fl.max())),
+                                            "rel_l2_median": sig(float(np.median(e_fl))),
+                                            "what": "FP64 with safety_f 0.79 vs 0.8"}}
+        tot["std"] += t_std
+        for pol, kw in POLICIES.items():
+            s.set_proposer(net, **kw)
+            u_l, st_l, t_l = run(s, ens, tf)
+            fam[pol] = entry(u_l, st_l, t_l)
+            tot[pol] = tot.get(pol, 0.0) + t_l
+        s.set_proposer(None)
+        out["families"][name] = fam
+    n_all = 2 * args.learned_count
+    out["combined"] = {k + "_traj_per_s": sig(n_all / v) for k, v in tot.items()}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -291,9 +376,12 @@ def main():
     ap.add_argument("--cpu-workers", type=int, default=0, help="0: min(16, cpu_count)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--modes", default="f64,mixed,f32", help="precision modes to run (f64 always runs)")
-    ap.add_argument("--workloads", default="nls,ks,ac2d")
+    ap.add_argument("--workloads", default="nls,ks,ac2d,learned")
     ap.add_argument("--ks-count", type=int, default=4096)
     ap.add_argument("--ac-batch", type=int, default=64)
+    ap.add_argument("--learned-count", type=int, default=1024, help="trajectories per family")
+    ap.add_argument("--learned-records", type=int, default=200000)
+    ap.add_argument("--learned-steps", type=int, default=2000)
     ap.add_argument("--copy-mib", type=int, default=0, help="device copy of this size first (counter scale)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -346,6 +434,8 @@ def main():
         workloads["ks_ensemble"] = ks_workload(args)
     if "ac2d" in wl:
         workloads["allen_cahn_2d"] = ac_workload(args)
+    if "learned" in wl:
+        workloads["learned_dt"] = learned_workload(args)
 
     line = {
         "metric": METRIC,

@@ -164,8 +164,13 @@ int rks_read_snapshots(rks_plan* p, int inst, void* dst_dev, double* t_host, int
  *   f_64 = ln h of the accepted step.
  *   f_65 = ln min(s, 1e4) of its attempt.
  * The network sees (f - mu) / sig.  Its output y estimates ln(h' s' / h), where
- * (h', s') is the next attempt.  The proposal is
- * h exp(clamp(y - margin, ln min_s, ln max_s)).
+ * (h', s') is the next attempt.  With d = y - margin, the next step is:
+ *   RKS_PROPOSE  h itself when keep_lo <= d < keep_hi (like the reference keeping
+ *                h for 1 <= s <= incr_f, this leaves the coefficient cache
+ *                valid), else h exp(clamp(d, ln min_s, ln max_s));
+ *   RKS_GUARD    the reference controller's proposal h_ref, unless
+ *                d < ln(h_ref / h) + keep_lo (its rejection is predicted); then
+ *                h exp(max(d, ln min_s)).
  *
  * Parameters: one float32 vector of RKS_PROPOSER_NPARAM values,
  *   W1 [256][96] (columns 66..95 are zero padding), b1 [256], W2 [256][256], b2,
@@ -175,6 +180,7 @@ int rks_read_snapshots(rks_plan* p, int inst, void* dst_dev, double* t_host, int
  * clamped to tf or its error was zero or non-finite.
  * ------------------------------------------------------------------------- */
 enum { RKS_PROPOSER_NIN = 66, RKS_PROPOSER_REC = 68, RKS_PROPOSER_NPARAM = 156673 };
+enum { RKS_PROPOSE = 0, RKS_GUARD = 1 };
 
 typedef struct rks_mlp rks_mlp;
 
@@ -202,7 +208,8 @@ int rks_mlp_infer_single(rks_mlp* net, const float* rec, int nrec, int threads, 
 /* Attach (net != NULL) or detach the proposer for the plan's evolves.  The plan
  * must be adaptive, 1-D built-in (n >= 256 rfft, n >= 128 fft).  The net must
  * outlive the plan's use of it. */
-int rks_plan_set_proposer(rks_plan* p, const rks_mlp* net, double margin);
+int rks_plan_set_proposer(rks_plan* p, const rks_mlp* net, int policy, double margin, double keep_lo,
+                          double keep_hi);
 /* Record up to rec_cap training records per instance during evolve (0 turns off). */
 int rks_set_records(rks_plan* p, int rec_cap);
 /* Synchronise; counts[i] (host, optional) = records of instance i; dst_dev

@@ -28,6 +28,7 @@ EXPORTS = (
 
 # learned step-size proposer (include/rkstiff_amd.h)
 PROPOSER_NIN, PROPOSER_REC, PROPOSER_NPARAM = 66, 68, 156673
+POLICIES = {"propose": 0, "guard": 1}
 
 
 class Config(ctypes.Structure):
@@ -138,7 +139,7 @@ def load(path=None):
         lib.rks_mlp_forward.argtypes = [vp, vp, ip, vp, vp]
         lib.rks_mlp_gradient.argtypes = [vp, vp, ip, ip, dp, vp, vp, vp, vp]
         lib.rks_mlp_infer_single.argtypes = [vp, vp, ip, ip, vp, vp]
-        lib.rks_plan_set_proposer.argtypes = [vp, vp, dp]
+        lib.rks_plan_set_proposer.argtypes = [vp, vp, ip, dp, dp, dp]
         lib.rks_set_records.argtypes = [vp, ip]
         lib.rks_read_records.argtypes = [vp, vp, P(ip), vp]
         lib.rks_last_error.restype = ctypes.c_char_p

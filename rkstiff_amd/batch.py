@@ -99,10 +99,20 @@ class BatchSolver:
         """(records (ninst, cap, 68) device tensor, records per trajectory)."""
         return self.plan.read_records()
 
-    def set_proposer(self, net, margin=0.0):
-        """Propose the step after each accept with ``net`` (a DtProposer); None restores the
-        reference's controller."""
-        self.plan.set_proposer(net, margin)
+    def set_proposer(self, net, margin=0.0, keep=None, policy="propose"):
+        """Set the step after each accept with ``net`` (a DtProposer); None restores the
+        reference's controller.  With d = y - margin (y: the network output):
+
+        "propose"  keep h (no coefficient regeneration) when keep[0] <= d < keep[1], else
+                   take h exp(d); the default band [0, ln incr_f) mirrors the reference
+                   keeping h while 1 <= s <= incr_f.
+        "guard"    the reference controller's proposal h_ref, unless d < ln(h_ref / h) + keep[0]
+                   (the network predicts its rejection): then take h exp(d) ahead of it.
+                   Default keep[0] = 0.
+        """
+        if keep is None:
+            keep = (0.0, float(np.log(self.config.incr_f)))
+        self.plan.set_proposer(net, margin, keep, policy)
 
     def stats(self):
         """Per-trajectory counters: tc, steps, attempts, rejects, status, ..."""

@@ -160,7 +160,7 @@ extern template __global__ void rks_kernel<double, float, 13, NLK_CUBIC>(rks_lau
 #include "rkstiff_amd.h"
 
 int rks_build_tableau(int method, rks_tableau* out);
-rks_mlp_dev rks_mlp_device(const rks_mlp* n, double margin);
+rks_mlp_dev rks_mlp_device(const rks_mlp* n, int policy, double margin, double keep_lo, double keep_hi);
 int rks_mlp_device_of(const rks_mlp* n);
 
 namespace {
@@ -836,7 +836,8 @@ static int learned_supported(const rks_plan* p) {
   return RKS_OK;
 }
 
-int rks_plan_set_proposer(rks_plan* p, const rks_mlp* net, double margin) {
+int rks_plan_set_proposer(rks_plan* p, const rks_mlp* net, int policy, double margin, double keep_lo,
+                          double keep_hi) {
   if (!p) return fail(RKS_ERR_INVALID, "null plan");
   if (!net) {
     p->mlp = rks_mlp_dev{};
@@ -845,8 +846,10 @@ int rks_plan_set_proposer(rks_plan* p, const rks_mlp* net, double margin) {
   int rc = learned_supported(p);
   if (rc) return rc;
   if (rks_mlp_device_of(net) != p->device) return fail(RKS_ERR_INVALID, "proposer and plan are on different devices");
-  if (!std::isfinite(margin)) return fail(RKS_ERR_INVALID, "margin must be finite");
-  p->mlp = rks_mlp_device(net, margin);
+  if (!std::isfinite(margin) || !std::isfinite(keep_lo) || !std::isfinite(keep_hi))
+    return fail(RKS_ERR_INVALID, "margin and keep band must be finite");
+  if (policy != RKS_PROPOSE && policy != RKS_GUARD) return fail(RKS_ERR_INVALID, "unknown proposer policy");
+  p->mlp = rks_mlp_device(net, policy, margin, keep_lo, keep_hi);
   return RKS_OK;
 }
 

@@ -133,12 +133,30 @@ __device__ __forceinline__ z64 frac_arg(z64 z, int num, int den, int cplx) {
   return mkc(w.re / (double)den, w.im / (double)den);
 }
 
-__device__ __forceinline__ z64 contour_node(const rks_ctrl& ct, int j) {
+__device__ __forceinline__ z64 contour_node(int npts, double radius, int j) {
   // R exp(2 pi i (j + 1/2) / N)   (etd35.py:259)
-  const double th = (6.283185307179586 * (j + 0.5)) * (1.0 / (double)ct.contour_points);
+  const double th = (6.283185307179586 * (j + 0.5)) * (1.0 / (double)npts);
   double sn, cs;
   sincos(th, &sn, &cs);
-  return mkc(ct.contour_radius * cs, ct.contour_radius * sn);
+  return mkc(radius * cs, radius * sn);
+}
+
+// The coefficient generator runs in a separate (noinline) function, where a
+// reference argument is a per-lane generic pointer: every read of the
+// tableau would be a vector memory load with full latency inside the
+// per-mode recipe loops.  The tableau and controller are uniform and
+// read-only during a launch, so the generator reads them through a
+// constant-address-space view of the uniform pointer (scalar loads).
+#define RKS_AS4 __attribute__((address_space(4)))
+typedef RKS_AS4 const rks_tableau rks_ctab;
+typedef RKS_AS4 const rks_ctrl rks_cctl;
+
+template <typename P>
+__device__ __forceinline__ P* rks_uniform_const(const void* p) {
+  const unsigned long long v = (unsigned long long)p;
+  const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)v);
+  const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(v >> 32));
+  return (P*)(((unsigned long long)hi << 32) | lo);
 }
 
 __device__ __forceinline__ z64 mode_z(const double* __restrict__ L, int L_complex, int m, double h) {
@@ -146,14 +164,14 @@ __device__ __forceinline__ z64 mode_z(const double* __restrict__ L, int L_comple
   return mkc(h * Lm.re, h * Lm.im);
 }
 
-// Basis values (exp, h psi1, h psi2, h psi3) of fraction f by the closed forms.
-__device__ __forceinline__ void frac_direct(const rks_frac& fr, z64 z, double h, int cplx, z64 (&v)[4]) {
+// Basis values (exp, h psi1, h psi2, h psi3) of fraction fnum/fden by the closed forms.
+__device__ __forceinline__ void frac_direct(int fnum, int fden, int mask, z64 z, double h, int cplx, z64 (&v)[4]) {
 #pragma clang fp contract(off)
-  const z64 e = np_exp(frac_arg(z, fr.fnum, fr.fden, cplx));
+  const z64 e = np_exp(frac_arg(z, fnum, fden, cplx));
   v[0] = e;
-  if (fr.mask & 14) {
+  if (mask & 14) {
     z64 p1, p2, p3 = mkc(0.0, 0.0);
-    np_psi123_e(frac_arg(z, fr.fnum, fr.fden, 1), e, fr.mask & 8, &p1, &p2, &p3);
+    np_psi123_e(frac_arg(z, fnum, fden, 1), e, mask & 8, &p1, &p2, &p3);
     v[1] = np_rmul(h, p1);
     v[2] = np_rmul(h, p2);
     v[3] = np_rmul(h, p3);
@@ -161,10 +179,10 @@ __device__ __forceinline__ void frac_direct(const rks_frac& fr, z64 z, double h,
 }
 
 // psi_r(f Z) summed into acc[1..3] at one contour point Z.
-__device__ __forceinline__ void frac_contour_add(const rks_frac& fr, z64 Z, z64 (&acc)[4]) {
+__device__ __forceinline__ void frac_contour_add(int fnum, int fden, int mask, z64 Z, z64 (&acc)[4]) {
 #pragma clang fp contract(off)
   z64 p1, p2, p3 = mkc(0.0, 0.0);
-  np_psi123(frac_arg(Z, fr.fnum, fr.fden, 1), fr.mask & 8, &p1, &p2, &p3);
+  np_psi123(frac_arg(Z, fnum, fden, 1), mask & 8, &p1, &p2, &p3);
   acc[1] = mkc(acc[1].re + p1.re, acc[1].im + p1.im);
   acc[2] = mkc(acc[2].re + p2.re, acc[2].im + p2.im);
   acc[3] = mkc(acc[3].re + p3.re, acc[3].im + p3.im);
@@ -172,12 +190,12 @@ __device__ __forceinline__ void frac_contour_add(const rks_frac& fr, z64 Z, z64 
 
 // Every coefficient recipe of fraction f: c = (outer * sum_k w_k B_k) / den,
 // stored at cbase + coef_off[c]*M in state (RS) or storage (RN) precision.
-template <typename RS, typename RN>
-__device__ __forceinline__ void frac_coefs(const rks_tableau& tb, int f, const z64 (&v)[4], double h, int cplx,
+template <typename RS, typename RN, typename TB>
+__device__ __forceinline__ void frac_coefs(TB& tb, int f, const z64 (&v)[4], double h, int cplx,
                                            int m, size_t M, char* __restrict__ cbase) {
 #pragma clang fp contract(off)
   for (int c = 0; c < tb.ncoef; ++c) {
-    const rks_coef& cf = tb.coef[c];
+    auto& cf = tb.coef[c];
     if (cf.frac != f) continue;
     z64 acc = mkc(0.0, 0.0);
     for (int k = 0; k < cf.nb; ++k) {
@@ -210,13 +228,13 @@ __device__ void coef_mode(const rks_tableau& tb, const rks_ctrl& ct, const doubl
     const rks_frac fr = tb.frac[f];
     z64 v[4];
     v[1] = v[2] = v[3] = mkc(0.0, 0.0);
-    frac_direct(fr, z, h, cplx, v);
+    frac_direct(fr.fnum, fr.fden, fr.mask, z, h, cplx, v);
     if (small && (fr.mask & 14)) {
       z64 acc[4];
       acc[1] = acc[2] = acc[3] = mkc(0.0, 0.0);
       for (int j = 0; j < ct.contour_points; ++j) {
-        const z64 nd = contour_node(ct, j);
-        frac_contour_add(fr, mkc(z.re + nd.re, z.im + nd.im), acc);
+        const z64 nd = contour_node(ct.contour_points, ct.contour_radius, j);
+        frac_contour_add(fr.fnum, fr.fden, fr.mask, mkc(z.re + nd.re, z.im + nd.im), acc);
       }
       for (int r = 1; r <= 3; ++r) {
         const z64 a = r == 1 ? acc[1] : (r == 2 ? acc[2] : acc[3]);
@@ -235,10 +253,12 @@ __device__ void coef_mode(const rks_tableau& tb, const rks_ctrl& ct, const doubl
 // (one lane per node, deterministic shuffle reduction).  `list` is LDS
 // scratch for up to M ints; *count is an LDS counter.
 template <typename RS, typename RN, int T, int MPT>
-__device__ __attribute__((noinline)) void coef_update(const rks_tableau& tb, const rks_ctrl& ct,
+__device__ __attribute__((noinline)) void coef_update(const rks_tableau& tb_in, const rks_ctrl& ct_in,
                                                       const double* __restrict__ L, int L_complex, int M, double h,
                                                       char* __restrict__ cbase, int* list, int* count) {
 #pragma clang fp contract(off)
+  rks_ctab& tb = *rks_uniform_const<rks_ctab>(&tb_in);
+  rks_cctl& ct = *rks_uniform_const<rks_cctl>(&ct_in);
   const int t = threadIdx.x;
   const int cplx = L_complex || tb.uses_contour;
   if (t == 0) *count = 0;
@@ -254,7 +274,7 @@ __device__ __attribute__((noinline)) void coef_update(const rks_tableau& tb, con
     for (int f = 0; f < tb.nfrac; ++f) {
       z64 v[4];
       v[1] = v[2] = v[3] = mkc(0.0, 0.0);
-      frac_direct(tb.frac[f], z, h, cplx, v);
+      frac_direct(tb.frac[f].fnum, tb.frac[f].fden, tb.frac[f].mask, z, h, cplx, v);
       frac_coefs<RS, RN>(tb, f, v, h, cplx, m, (size_t)M, cbase);
     }
   }
@@ -270,15 +290,15 @@ __device__ __attribute__((noinline)) void coef_update(const rks_tableau& tb, con
         const int m = list[idx];
         const z64 z = mode_z(L, L_complex, m, h);
         for (int f = 0; f < tb.nfrac; ++f) {
-          const rks_frac fr = tb.frac[f];
+          const int fnum = tb.frac[f].fnum, fden = tb.frac[f].fden, fmask = tb.frac[f].mask;
           z64 v[4];
           v[1] = v[2] = v[3] = mkc(0.0, 0.0);
-          if (fr.mask & 14) {
+          if (fmask & 14) {
             z64 acc[4];
             acc[1] = acc[2] = acc[3] = mkc(0.0, 0.0);
             for (int j = lane; j < ct.contour_points; j += GS) {
-              const z64 nd = contour_node(ct, j);
-              frac_contour_add(fr, mkc(z.re + nd.re, z.im + nd.im), acc);
+              const z64 nd = contour_node(ct.contour_points, ct.contour_radius, j);
+              frac_contour_add(fnum, fden, fmask, mkc(z.re + nd.re, z.im + nd.im), acc);
             }
 #pragma unroll
             for (int o = GS / 2; o >= 1; o >>= 1) {
@@ -293,7 +313,7 @@ __device__ __attribute__((noinline)) void coef_update(const rks_tableau& tb, con
             v[3] = np_rdiv(np_rmul(h, acc[3]), (double)ct.contour_points);
           }
           if (lane == 0) {
-            v[0] = np_exp(frac_arg(z, fr.fnum, fr.fden, cplx));
+            v[0] = np_exp(frac_arg(z, fnum, fden, cplx));
             frac_coefs<RS, RN>(tb, f, v, h, cplx, m, (size_t)M, cbase);
           }
         }
@@ -652,8 +672,13 @@ __device__ __forceinline__ void eval_nl(GetK getk, cx<R>* buf, const cx<R>* twl,
 // (fft layout) and lies in bin (64 w) / (n/2 + 1); f_64 = ln h of the accepted
 // step, f_65 = ln min(s, 1e4) of its attempt.  |u_hat| comes from the LDS
 // buffer where the fused final stage left it, else from the candidate array.
-// Returns the next step: the proposal h exp(clamp(y - margin)) when a
-// proposer is attached and its output is finite, else h_suggest.
+// Returns the next step when a proposer is attached and its output y is
+// finite.  Policy "propose": h itself when keep_lo <= y - margin < keep_hi
+// (no coefficient regeneration), else h exp(clamp(y - margin, ln min_s,
+// ln max_s)).  Policy "guard": h_suggest (the reference controller's
+// proposal) unless y - margin < ln(h_suggest / h) + keep_lo, i.e. the network
+// predicts that proposal's rejection; then h exp(max(y - margin, ln min_s)).
+// Without a proposer or with a non-finite y: h_suggest.
 template <typename R, int LOG2N, int T, int NLK>
 __device__ __attribute__((noinline)) double learned_after_accept(const rks_launch& P, const rks_ctrl& ct, rks_inst* in,
                                        const cx<R>* ucand, const cx<R>* buf, bool fused, double h_used,
@@ -705,9 +730,18 @@ __device__ __attribute__((noinline)) double learned_after_accept(const rks_launc
     const float y = rks_mlp_infer<T>(P.mlp, scr, scr + RKS_MLP_K0);
     if (!(isinf(y) || isnan(y))) {
       double yl = (double)y - (double)P.mlp.margin;
-      const double lo = log(ct.min_s), hi = log(ct.max_s);
-      yl = yl < lo ? lo : (yl > hi ? hi : yl);
-      h_suggest = h_used * exp(yl);
+      if (P.mlp.policy == 1) {
+        if (yl < log(h_suggest / h_used) + (double)P.mlp.keep_lo) {
+          const double lo = log(ct.min_s);
+          h_suggest = h_used * exp(yl < lo ? lo : yl);
+        }
+      } else if (yl >= (double)P.mlp.keep_lo && yl < (double)P.mlp.keep_hi) {
+        h_suggest = h_used;
+      } else {
+        const double lo = log(ct.min_s), hi = log(ct.max_s);
+        yl = yl < lo ? lo : (yl > hi ? hi : yl);
+        h_suggest = h_used * exp(yl);
+      }
     }
   }
   return h_suggest;

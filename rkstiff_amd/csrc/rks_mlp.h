@@ -6,8 +6,9 @@
 // Weights are BF16, row-major [out][in]; the hidden layers multiply on the
 // matrix cores with v_mfma_f32_16x16x32_bf16 and FP32 accumulation.  The
 // output y estimates ln(h' s' / h), the log ratio between the largest step
-// that the next attempt would accept and the step just accepted; the
-// proposal is h * exp(clamp(y - margin, ln min_s, ln max_s)).
+// that the next attempt would accept and the step just accepted.  The step
+// is kept when y - margin falls in a keep band (the coefficient cache stays
+// valid), else it becomes h * exp(clamp(y - margin, ln min_s, ln max_s)).
 //
 // The reference has no learned part (its controller, solveras.py:543-554,
 // keeps h unless s > incr_f); this is an opt-in addition.  The embedded
@@ -46,6 +47,14 @@ struct rks_mlp_dev {
   const float* sig;
   float margin;         // subtracted from the output before the clamp
   int on;
+  // policy 0 (propose): the step is left unchanged (no coefficient
+  //   regeneration) when keep_lo <= y - margin < keep_hi, as the reference keeps
+  //   h for 1 <= s <= incr_f, and follows the network otherwise;
+  // policy 1 (guard): the reference controller's proposal h_ref stands unless
+  //   y - margin < ln(h_ref / h) + keep_lo (the network predicts its
+  //   rejection), then the step becomes h exp(y - margin) ahead of it.
+  float keep_lo, keep_hi;
+  int policy;
 };
 
 typedef __bf16 rks_bf16x8 __attribute__((ext_vector_type(8)));
@@ -100,33 +109,52 @@ __device__ inline rks_bf16x8 rks_pack_bf16x8(const float* x) {
 // columns are zero); each 16-row tile of W is the A operand, so a wave
 // produces 16 outputs per K/32 instructions.  Column 0 of D sits in lanes
 // 0, 16, 32, 48 (rows 4 (lane/16) + j).
+// Batch 1 leaves the layer bound by the latency of streaming W (128 KB)
+// from the caches, so a wave loads the fragments of G tiles (G K/32 loads in
+// flight) before it multiplies them, and builds the input fragments once.
 template <int T, int K>
 __device__ inline void rks_mlp_layer1(const uint16_t* __restrict__ W, const float* __restrict__ b,
                                       const float* in, float* out) {
   static_assert(T % 64 == 0, "whole waves");
   constexpr int NW = T / 64;
+  constexpr int KT = K / 32;
+  constexpr int NT = (RKS_MLP_HID / 16) / NW;  // tiles per wave
+  constexpr int GW = KT <= 3 ? 4 : 2;
+  constexpr int G = NT < GW ? NT : GW;         // tiles per load group
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int col = lane & 15, kq = lane >> 4;
-  for (int ot = wave; ot < RKS_MLP_HID / 16; ot += NW) {
-    rks_f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-    const uint16_t* wrow = W + (size_t)(ot * 16 + col) * K + 8 * kq;
-#pragma unroll 4
-    for (int kt = 0; kt < K / 32; ++kt) {
-      const rks_bf16x8 a = rks_load_bf16x8(wrow + kt * 32);
-      rks_bf16x8 bv;
-      if (col == 0) {
-        bv = rks_pack_bf16x8(in + kt * 32 + 8 * kq);
-      } else {
-        const uint4 z = {0u, 0u, 0u, 0u};
-        bv = __builtin_bit_cast(rks_bf16x8, z);
-      }
-      acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bv, acc, 0, 0, 0);
-    }
-    if (col == 0) {
+  rks_bf16x8 bv[KT];
 #pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        const int r = ot * 16 + 4 * kq + j;
-        out[r] = rks_bfround(rks_gelu(acc[j] + b[r]));
+  for (int kt = 0; kt < KT; ++kt) {
+    if (col == 0) {
+      bv[kt] = rks_pack_bf16x8(in + kt * 32 + 8 * kq);
+    } else {
+      const uint4 z = {0u, 0u, 0u, 0u};
+      bv[kt] = __builtin_bit_cast(rks_bf16x8, z);
+    }
+  }
+#pragma unroll 1
+  for (int g0 = 0; g0 < NT; g0 += G) {
+    rks_bf16x8 a[G][KT];
+#pragma unroll
+    for (int i = 0; i < G; ++i) {
+      const int ot = wave + (g0 + i) * NW;
+      const uint16_t* wrow = W + (size_t)(ot * 16 + col) * K + 8 * kq;
+#pragma unroll
+      for (int kt = 0; kt < KT; ++kt) a[i][kt] = rks_load_bf16x8(wrow + kt * 32);
+    }
+#pragma unroll
+    for (int i = 0; i < G; ++i) {
+      const int ot = wave + (g0 + i) * NW;
+      rks_f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int kt = 0; kt < KT; ++kt) acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i][kt], bv[kt], acc, 0, 0, 0);
+      if (col == 0) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const int r = ot * 16 + 4 * kq + j;
+          out[r] = rks_bfround(rks_gelu(acc[j] + b[r]));
+        }
       }
     }
   }

@@ -27,7 +27,7 @@ static_assert(RKS_PROPOSER_NIN == RKS_MLP_NIN && RKS_PROPOSER_REC == RKS_MLP_REC
               "public proposer sizes");
 
 void rks_set_last_error(const std::string& msg);  // rks_host.hip: the library's one error channel
-rks_mlp_dev rks_mlp_device(const rks_mlp* n, double margin);
+rks_mlp_dev rks_mlp_device(const rks_mlp* n, int policy, double margin, double keep_lo, double keep_hi);
 
 namespace {
 
@@ -556,7 +556,7 @@ int rks_mlp_gradient(rks_mlp* n, const float* rec, int nrec, int loss_kind, doub
 int rks_mlp_infer_single(rks_mlp* n, const float* rec, int nrec, int threads, float* yhat, hipStream_t s) {
   if (!n || !rec || !yhat || nrec < 0) return mfail(RKS_ERR_INVALID, "bad argument");
   if (nrec == 0) return RKS_OK;
-  rks_mlp_dev d = rks_mlp_device(n, 0.0);
+  rks_mlp_dev d = rks_mlp_device(n, 0, 0.0, 0.0, 0.0);
   switch (threads) {
     case 64: hipLaunchKernelGGL(mlp_infer_single<64>, dim3(nrec), dim3(64), 0, s, d, rec, nrec, yhat); break;
     case 128: hipLaunchKernelGGL(mlp_infer_single<128>, dim3(nrec), dim3(128), 0, s, d, rec, nrec, yhat); break;
@@ -571,7 +571,7 @@ int rks_mlp_infer_single(rks_mlp* n, const float* rec, int nrec, int threads, fl
 }  // extern "C"
 
 // device view of a proposer for the stepping kernels (rks_host.hip)
-rks_mlp_dev rks_mlp_device(const rks_mlp* n, double margin) {
+rks_mlp_dev rks_mlp_device(const rks_mlp* n, int policy, double margin, double keep_lo, double keep_hi) {
   rks_mlp_dev d{};
   if (!n) return d;
   d.wb = n->d_wb;
@@ -579,6 +579,9 @@ rks_mlp_dev rks_mlp_device(const rks_mlp* n, double margin) {
   d.mu = n->d_mu;
   d.sig = n->d_sig;
   d.margin = (float)margin;
+  d.keep_lo = (float)keep_lo;
+  d.keep_hi = (float)keep_hi;
+  d.policy = policy;
   d.on = 1;
   return d;
 }

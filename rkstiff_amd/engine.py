@@ -131,8 +131,12 @@ class Plan:
         _lib.check(self.lib.rks_read_records(self.h, ctypes.c_void_p(out.data_ptr()), counts, _stream()))
         return out, list(counts)
 
-    def set_proposer(self, net, margin=0.0):
-        _lib.check(self.lib.rks_plan_set_proposer(self.h, net.h if net is not None else None, float(margin)))
+    def set_proposer(self, net, margin=0.0, keep=(0.0, 0.0), policy="propose"):
+        if policy not in _lib.POLICIES:
+            raise ValueError(f"policy must be one of {sorted(_lib.POLICIES)}")
+        _lib.check(self.lib.rks_plan_set_proposer(self.h, net.h if net is not None else None,
+                                                  _lib.POLICIES[policy], float(margin), float(keep[0]),
+                                                  float(keep[1])))
         self.proposer = net  # the plan's kernels read the network's device buffers
 
     # -- launches --------------------------------------------------------------

@@ -239,3 +239,63 @@ def test_proposer_rejects_unsupported_plans():
     bad[5] = np.nan
     with pytest.raises(ValueError):
         DtProposer(params=bad)
+
+
+def test_learned_proposer_gain():
+    """What the trained network gains over the reference controller (KS, IF34, FP32).
+
+    Records come from the solver's own runs on one seeded ensemble; the gain is
+    measured on another.  The network must cut rejections and attempts at an
+    accuracy vs FP64 that stays at the level of the FP64 step-sequence floor.
+    Wall times are printed, not asserted: every change of h regenerates the
+    coefficient arrays and every accepted step runs one batch-1 inference.
+    """
+    import time as _time
+    eps, tf = 1e-5, 20.0
+    train = ks_ensemble(128, seed=31, sizes=(1024, 2048, 4096))
+    test = ks_ensemble(64, seed=32, sizes=(1024, 2048, 4096))
+
+    def mk(ens, prec, safety=0.8):
+        return BatchSolver("IF34", [(torch.as_tensor(L), nl) for (_, _, L, nl, _) in ens], precision=prec,
+                           config=SolverConfig(epsilon=eps, safety_f=safety))
+
+    def run(s, ens):
+        u0 = [torch.as_tensor(e[4]).to(s.dtype).cuda() for e in ens]
+        torch.cuda.synchronize()
+        t = _time.perf_counter()
+        uf = s.evolve(u0, 0.0, tf)
+        torch.cuda.synchronize()
+        st = s.stats()
+        return uf, st, _time.perf_counter() - t
+
+    def err(a, b):
+        return np.array([float(torch.linalg.norm(x.to(torch.complex128) - y) / torch.linalg.norm(y))
+                         for x, y in zip(a, b)])
+
+    s = mk(train, "f32")
+    s.record(3000)
+    run(s, train)
+    data = training_set(*s.records(), s.plan.cfg)
+    assert data.shape[0] > 20000
+    net = DtProposer.for_data(data, seed=0)
+    hist = net.train(data, steps=800, batch=1024, lr=1e-3, tau=0.2)
+    assert hist[-50:].mean() < 0.5 * hist[:10].mean()
+
+    ref, _, _ = run(mk(test, "f64"), test)
+    floor, _, _ = run(mk(test, "f64", safety=0.79), test)
+    s = mk(test, "f32")
+    u_std, st_std, t_std = run(s, test)
+    s.set_proposer(net, margin=0.02)
+    u_l, st_l, t_l = run(s, test)
+    tot = lambda st, k: sum(x[k] for x in st)
+    a_std, a_l = tot(st_std, "attempts"), tot(st_l, "attempts")
+    r_std, r_l = tot(st_std, "rejects"), tot(st_l, "rejects")
+    e_std, e_l, e_fl = err(u_std, ref), err(u_l, ref), err(floor, ref)
+    print(f"attempts {a_l} vs {a_std} (x{a_l / a_std:.3f}), rejects {r_l} vs {r_std}, "
+          f"coef updates {tot(st_l, 'coef_updates')} vs {tot(st_std, 'coef_updates')}, "
+          f"time {t_l:.3f} vs {t_std:.3f} s; err max {e_l.max():.2e} vs {e_std.max():.2e} "
+          f"(floor {e_fl.max():.2e})")
+    assert all(x["status"] == _lib.RKS_OK for x in st_l)
+    assert r_l <= 0.5 * r_std
+    assert a_l <= 0.98 * a_std
+    assert e_l.max() <= 10.0 * max(e_fl.max(), e_std.max())
