@@ -216,6 +216,23 @@ int rks_set_records(rks_plan* p, int rec_cap);
  * (optional, device) receives ninst x rec_cap x RKS_PROPOSER_REC floats. */
 int rks_read_records(rks_plan* p, float* dst_dev, int* counts, hipStream_t stream);
 
+/* ---------------------------------------------------------------------------
+ * Quasi-discrete Hankel transform of order zero: HankelTransform.ht / .iht
+ * (transforms.py:172-240) and the radial cubic NLS term of demos/nls.ipynb
+ * cell 8.  Y is the reference's nr x nr matrix 2 J0(j_i j_j / j_{N+1}) /
+ * (j_{N+1} J1(j_j)^2) (transforms.py:113-117, row-major float64, host or
+ * device, copied); scale is S = rmax^2 / j_{N+1}.  Vectors are rows of nr
+ * values (complex128 interleaved when cplx, else float64), device pointers.
+ * Calls on one transform object must not overlap (it keeps one scratch).
+ * ------------------------------------------------------------------------- */
+typedef struct rks_hankel rks_hankel;
+int rks_hankel_create(int nr, const double* Y, double scale, rks_hankel** out);
+int rks_hankel_destroy(rks_hankel* h);
+/* out[b] = S Y in[b] (forward, ht) or Y in[b] / S (inverse, iht), b < batch. */
+int rks_hankel_apply(rks_hankel* h, int inverse, const void* in, void* out, int batch, int cplx, hipStream_t stream);
+/* out[b] = 1j gamma ht(|f|^2 f), f = iht(in[b]) (complex128 only). */
+int rks_hankel_cubic(rks_hankel* h, double gamma, const void* in, void* out, int batch, hipStream_t stream);
+
 /* Method tableau (host-only, no device needed): fills `bytes` >= rks_sizeof(0). */
 int rks_get_tableau(int method, void* out, int bytes);
 int rks_sizeof(int what);

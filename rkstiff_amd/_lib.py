@@ -23,7 +23,7 @@ EXPORTS = (
     "rks_sizeof", "rks_last_error", "rks_set_config",
     "rks_mlp_create", "rks_mlp_destroy", "rks_mlp_get_params", "rks_mlp_train", "rks_mlp_forward",
     "rks_mlp_gradient", "rks_mlp_infer_single", "rks_plan_set_proposer", "rks_set_records",
-    "rks_read_records",
+    "rks_read_records", "rks_hankel_create", "rks_hankel_destroy", "rks_hankel_apply", "rks_hankel_cubic",
 )
 
 # learned step-size proposer (include/rkstiff_amd.h)
@@ -142,6 +142,10 @@ def load(path=None):
         lib.rks_plan_set_proposer.argtypes = [vp, vp, ip, dp, dp, dp]
         lib.rks_set_records.argtypes = [vp, ip]
         lib.rks_read_records.argtypes = [vp, vp, P(ip), vp]
+        lib.rks_hankel_create.argtypes = [ip, P(dp), dp, P(vp)]
+        lib.rks_hankel_destroy.argtypes = [vp]
+        lib.rks_hankel_apply.argtypes = [vp, ip, vp, vp, ip, ip, vp]
+        lib.rks_hankel_cubic.argtypes = [vp, dp, vp, vp, ip, vp]
         lib.rks_last_error.restype = ctypes.c_char_p
         for name in EXPORTS:
             if name not in ("rks_last_error",):

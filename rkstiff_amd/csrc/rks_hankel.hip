@@ -1,0 +1,192 @@
+// Quasi-discrete Hankel transform of order zero on the device.
+//
+// The reference applies an nr x nr matrix Y of J0 values (transforms.py:
+// 113-117) as ht(f) = S Y f and iht(g) = Y g / S with S = rmax^2 / j_{N+1}
+// (transforms.py:172-240); its radial NLS demo (demos/nls.ipynb cell 8)
+// evaluates 1j gamma ht(|f|^2 f), f = iht(g), at every stage.
+//
+// Here one launch multiplies Y by a batch of vectors: out[b] = c Y in[b].
+// Tiles of 64 rows x 16 vectors with 32-wide K chunks staged in LDS, FP64
+// FMAs on the vector units (on MI355X the FP64 matrix-core rate equals the
+// FP64 vector rate, so MFMA would not add throughput here).  The epilogue
+// scales, or forms |f|^2 f (after the inverse), or multiplies by i gamma (after
+// the forward), so the radial NLS term is two launches.
+#include <hip/hip_runtime.h>
+
+#include <string>
+
+#include "rkstiff_amd.h"
+
+void rks_set_last_error(const std::string& msg);  // rks_host.hip
+
+namespace {
+
+constexpr int HB_M = 64;   // rows per tile
+constexpr int HB_B = 16;   // vectors per tile
+constexpr int HB_K = 32;   // K chunk
+
+enum { EPI_SCALE = 0, EPI_CUBIC = 1, EPI_TIMES_I = 2 };
+
+int hfail(int code, const std::string& msg) {
+  rks_set_last_error(msg);
+  return code;
+}
+
+// out[b][i] = epi(c * sum_j Y[i][j] in[b][j]); complex vectors (CPLX) are
+// interleaved (re, im), real vectors are plain doubles.
+template <bool CPLX, int EPI>
+__global__ __launch_bounds__(256) void hankel_gemm(const double* __restrict__ Y, int nr,
+                                                   const double* __restrict__ in, double* __restrict__ out,
+                                                   int batch, double c, double gamma) {
+  __shared__ double Ys[HB_M][HB_K + 1];
+  __shared__ double Xr[HB_K][HB_B];
+  __shared__ double Xi[HB_K][HB_B];
+  const int t = threadIdx.x;
+  const int tb = t % HB_B, ti = t / HB_B;  // 16 x 16 threads: vector, row group
+  const int i0 = blockIdx.x * HB_M, b0 = blockIdx.y * HB_B;
+  double ar[4] = {0.0, 0.0, 0.0, 0.0}, ai[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int j0 = 0; j0 < nr; j0 += HB_K) {
+#pragma unroll
+    for (int e = 0; e < HB_M * HB_K / 256; ++e) {
+      const int idx = t + 256 * e;
+      const int ii = idx / HB_K, jj = idx % HB_K;
+      const int i = i0 + ii, j = j0 + jj;
+      Ys[ii][jj] = (i < nr && j < nr) ? Y[(size_t)i * nr + j] : 0.0;
+    }
+#pragma unroll
+    for (int e = 0; e < HB_K * HB_B / 256; ++e) {
+      const int idx = t + 256 * e;
+      const int bb = idx / HB_K, jj = idx % HB_K;
+      const int b = b0 + bb, j = j0 + jj;
+      const bool ok = b < batch && j < nr;
+      if (CPLX) {
+        Xr[jj][bb] = ok ? in[2 * ((size_t)b * nr + j)] : 0.0;
+        Xi[jj][bb] = ok ? in[2 * ((size_t)b * nr + j) + 1] : 0.0;
+      } else {
+        Xr[jj][bb] = ok ? in[(size_t)b * nr + j] : 0.0;
+      }
+    }
+    __syncthreads();
+#pragma unroll 8
+    for (int jj = 0; jj < HB_K; ++jj) {
+      const double xr = Xr[jj][tb];
+      const double xi = CPLX ? Xi[jj][tb] : 0.0;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const double y = Ys[ti + 16 * r][jj];
+        ar[r] = fma(y, xr, ar[r]);
+        if (CPLX) ai[r] = fma(y, xi, ai[r]);
+      }
+    }
+    __syncthreads();
+  }
+  const int b = b0 + tb;
+  if (b >= batch) return;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int i = i0 + ti + 16 * r;
+    if (i >= nr) continue;
+    double vr = c * ar[r], vi = c * ai[r];
+    if (EPI == EPI_CUBIC) {
+      const double m2 = vr * vr + vi * vi;
+      vr *= m2;
+      vi *= m2;
+    } else if (EPI == EPI_TIMES_I) {
+      const double w = vr;
+      vr = -gamma * vi;
+      vi = gamma * w;
+    }
+    if (CPLX) {
+      out[2 * ((size_t)b * nr + i)] = vr;
+      out[2 * ((size_t)b * nr + i) + 1] = vi;
+    } else {
+      out[(size_t)b * nr + i] = vr;
+    }
+  }
+}
+
+}  // namespace
+
+struct rks_hankel {
+  int nr = 0, device = 0;
+  double scale = 1.0;
+  double* d_Y = nullptr;
+  double* d_tmp = nullptr;  // |f|^2 f of the cubic term
+  int tmp_batch = 0;
+};
+
+extern "C" {
+
+int rks_hankel_create(int nr, const double* Y, double scale, rks_hankel** out) {
+  if (!out || !Y) return hfail(RKS_ERR_INVALID, "null argument");
+  *out = nullptr;
+  if (nr < 4) return hfail(RKS_ERR_INVALID, "nr must be >= 4");
+  if (!(scale > 0.0)) return hfail(RKS_ERR_INVALID, "scale must be positive");
+  rks_hankel* h = new rks_hankel();
+  h->nr = nr;
+  h->scale = scale;
+  hipError_t e = hipGetDevice(&h->device);
+  if (e == hipSuccess) e = hipMalloc(&h->d_Y, sizeof(double) * (size_t)nr * nr);
+  if (e == hipSuccess) e = hipMemcpy(h->d_Y, Y, sizeof(double) * (size_t)nr * nr, hipMemcpyDefault);
+  if (e != hipSuccess) {
+    rks_hankel_destroy(h);
+    return hfail(RKS_ERR_HIP, std::string("rks_hankel_create: ") + hipGetErrorString(e));
+  }
+  *out = h;
+  return RKS_OK;
+}
+
+int rks_hankel_destroy(rks_hankel* h) {
+  if (!h) return RKS_OK;
+  (void)hipDeviceSynchronize();
+  (void)hipFree(h->d_Y);
+  (void)hipFree(h->d_tmp);
+  delete h;
+  return RKS_OK;
+}
+
+int rks_hankel_apply(rks_hankel* h, int inverse, const void* in, void* out, int batch, int cplx,
+                     hipStream_t stream) {
+  if (!h || !in || !out) return hfail(RKS_ERR_INVALID, "null argument");
+  if (batch < 0) return hfail(RKS_ERR_INVALID, "batch must be >= 0");
+  if (in == out) return hfail(RKS_ERR_INVALID, "in and out must not alias");
+  if (batch == 0) return RKS_OK;
+  const double c = inverse ? 1.0 / h->scale : h->scale;
+  dim3 grid((h->nr + HB_M - 1) / HB_M, (batch + HB_B - 1) / HB_B);
+  if (cplx)
+    hipLaunchKernelGGL((hankel_gemm<true, EPI_SCALE>), grid, dim3(256), 0, stream, h->d_Y, h->nr,
+                       (const double*)in, (double*)out, batch, c, 0.0);
+  else
+    hipLaunchKernelGGL((hankel_gemm<false, EPI_SCALE>), grid, dim3(256), 0, stream, h->d_Y, h->nr,
+                       (const double*)in, (double*)out, batch, c, 0.0);
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return hfail(RKS_ERR_HIP, std::string("rks_hankel_apply: ") + hipGetErrorString(e));
+  return RKS_OK;
+}
+
+int rks_hankel_cubic(rks_hankel* h, double gamma, const void* in, void* out, int batch, hipStream_t stream) {
+  if (!h || !in || !out) return hfail(RKS_ERR_INVALID, "null argument");
+  if (batch < 0) return hfail(RKS_ERR_INVALID, "batch must be >= 0");
+  if (in == out) return hfail(RKS_ERR_INVALID, "in and out must not alias");
+  if (batch == 0) return RKS_OK;
+  if (batch > h->tmp_batch) {
+    const hipError_t se = hipStreamSynchronize(stream);
+    if (se != hipSuccess) return hfail(RKS_ERR_HIP, std::string("rks_hankel_cubic: ") + hipGetErrorString(se));
+    (void)hipFree(h->d_tmp);
+    h->d_tmp = nullptr;
+    h->tmp_batch = 0;
+    const hipError_t e = hipMalloc(&h->d_tmp, sizeof(double) * 2 * (size_t)h->nr * batch);
+    if (e != hipSuccess) return hfail(RKS_ERR_HIP, std::string("rks_hankel_cubic: ") + hipGetErrorString(e));
+    h->tmp_batch = batch;
+  }
+  dim3 grid((h->nr + HB_M - 1) / HB_M, (batch + HB_B - 1) / HB_B);
+  hipLaunchKernelGGL((hankel_gemm<true, EPI_CUBIC>), grid, dim3(256), 0, stream, h->d_Y, h->nr,
+                     (const double*)in, h->d_tmp, batch, 1.0 / h->scale, 0.0);
+  hipLaunchKernelGGL((hankel_gemm<true, EPI_TIMES_I>), grid, dim3(256), 0, stream, h->d_Y, h->nr,
+                     (const double*)h->d_tmp, (double*)out, batch, h->scale, gamma);
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return hfail(RKS_ERR_HIP, std::string("rks_hankel_cubic: ") + hipGetErrorString(e));
+  return RKS_OK;
+}
+
+}  // extern "C"

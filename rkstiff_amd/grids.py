@@ -63,3 +63,48 @@ def construct_x_dx_cheb(n, a=-1.0, b=1.0, device=None):
     if device:
         return xt.to(device), dt.to(device)
     return xt, dt
+
+
+def construct_r_kr_hankel(nr, rmax, device=None):
+    """(r, kr, bessel_zeros, jN) of grids.py:222-264: with j_1 < ... < j_{nr+1} the zeros of
+    J0, r_i = j_i rmax / j_{nr+1} and kr_i = j_i / rmax (i <= nr); jN = j_{nr+1}."""
+    import scipy.special as sp
+    if nr < 4:
+        raise ValueError("nr must be ≥ 4.")
+    if rmax <= 0:
+        raise ValueError("rmax must be positive.")
+    z = sp.jn_zeros(0, nr + 1)
+    zeros, jN = z[:-1], float(z[-1])
+    r = torch.as_tensor(zeros * rmax / jN, dtype=torch.float64)
+    kr = torch.as_tensor(zeros / rmax, dtype=torch.float64)
+    zt = torch.as_tensor(zeros, dtype=torch.float64)
+    if device:
+        r, kr, zt = r.to(device), kr.to(device), zt.to(device)
+    return r, kr, zt, jN
+
+
+def mirror_grid(r, u=None, axis=-1):
+    """Mirror a radial grid (and optionally a function on it) onto [-rmax, rmax]
+    (grids.py:267-321): rnew = [-flip(r), r]; u is mirrored along ``axis``
+    (-1: 1-D, 0: rows, 1: columns).  Accepts NumPy arrays or torch tensors."""
+    if isinstance(r, torch.Tensor):
+        rnew = torch.cat([-torch.flip(r, dims=[0]), r])
+        if u is None:
+            return rnew, None
+        if axis == -1:
+            return rnew, torch.cat([torch.flip(u, dims=[-1]), u], dim=-1)
+        if axis == 0:
+            return rnew, torch.cat([torch.flip(u, dims=[0]), u], dim=0)
+        if axis == 1:
+            return rnew, torch.cat([torch.flip(u, dims=[1]), u], dim=1)
+        raise ValueError("axis must be -1, 0, or 1.")
+    rnew = np.hstack([-np.flipud(r), r])
+    if u is None:
+        return rnew, None
+    if axis == -1:
+        return rnew, np.hstack([np.flipud(u), u])
+    if axis == 0:
+        return rnew, np.vstack([np.flipud(u), u])
+    if axis == 1:
+        return rnew, np.hstack([np.fliplr(u), u])
+    raise ValueError("axis must be -1, 0, or 1.")

@@ -85,6 +85,17 @@ class BaseSolver:
         self.logger.setLevel(_level(loglevel))
 
     # -- plans -----------------------------------------------------------------
+    @staticmethod
+    def _as_state(u):
+        """A real state is promoted to complex (float64 -> complex128, float32 -> complex64),
+        as NumPy promotes it at the first complex stage of the reference."""
+        if isinstance(u, torch.Tensor) and u.is_cuda and not u.is_complex():
+            if u.dtype == torch.float64:
+                return u.to(torch.complex128)
+            if u.dtype == torch.float32:
+                return u.to(torch.complex64)
+        return u
+
     def _precision(self, u):
         if not isinstance(u, torch.Tensor) or not u.is_cuda:
             raise TypeError("rkstiff_amd solvers step device (HIP) tensors")
@@ -435,6 +446,7 @@ class BaseSolverAS(BaseSolver):
         """One accepted step: (u_new, h_used, h_suggest) (solveras.py:336-410)."""
         h = float(h_suggest)
         assert h >= 0.0
+        u = self._as_state(u)
         if self._builtin:
             return self._step_engine(u, h)
         return self._step_generic(u, h)
@@ -513,6 +525,7 @@ class BaseSolverAS(BaseSolver):
             h_init = (tf - t0) / 100.0
         if not tf > t0:  # no step is taken (solveras.py:624): u comes back unchanged
             return u
+        u = self._as_state(u)
         if self._builtin:
             return self._evolve_engine(u, float(t0), float(tf), float(h_init), store_data,
                                        int(store_freq))
@@ -605,6 +618,7 @@ class BaseSolverCS(BaseSolver):
         """One step of size h (solvercs.py:163-186)."""
         assert h >= 0.0
         h = float(h)
+        u = self._as_state(u)
         if self._builtin:
             plan = self._plan(u)
             u = u.contiguous()
@@ -624,6 +638,7 @@ class BaseSolverCS(BaseSolver):
             self.u.append(u)
         if t0 + h > tf:
             raise ValueError("Step size h must be <= (tf - t0); reduce h or extend tf.")
+        u = self._as_state(u)
         if not self._builtin:
             tc, step_count = t0, 0
             while tc < tf:

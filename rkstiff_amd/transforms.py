@@ -1,0 +1,141 @@
+"""Quasi-discrete Hankel transform of order zero on device tensors.
+
+``HankelTransform`` keeps the reference's API (transforms.py:43-240): the
+grids ``r`` and ``kr``, ``ht`` / ``iht``, ``hankel_matrix()``,
+``bessel_zeros()`` and the validated ``nr`` / ``rmax`` properties.  The J0
+matrix is built on the host in FP64, in the reference's operation order
+(transforms.py:113-117):
+
+    Y_ij = 2 J0(j_i j_j / j_{N+1}) / (j_{N+1} J1(j_j)^2),   S = rmax^2 / j_{N+1}.
+
+``ht(f) = S Y f`` and ``iht(g) = Y g / S`` run in librkstiff_amd.so (batched
+FP64 kernel, rkstiff_amd/csrc/rks_hankel.hip) on float64 or complex128
+tensors of shape (..., nr).  ``cubic_nl(gamma)`` is the radial NLS
+nonlinearity of demos/nls.ipynb cell 8, 1j gamma ht(|f|^2 f) with f = iht(g),
+evaluated by two fused launches.
+"""
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib
+from .grids import construct_r_kr_hankel
+
+
+def _stream():
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+class HankelTransform:
+    """Discrete Hankel transform of order zero on nr points of [0, rmax]."""
+
+    def __init__(self, nr, rmax=1.0, device="cuda"):
+        self._device = device
+        self._h = None
+        self._setnr_setrmax(nr, rmax)
+
+    def _setnr_setrmax(self, nr, rmax):
+        import scipy.special as sp
+        r, kr, zeros, jN = construct_r_kr_hankel(nr, rmax)
+        z = zeros.numpy()
+        Y = 2 * sp.j0(np.outer(z, z) / jN) / (jN * sp.j1(z) ** 2)
+        lib = _lib.load()
+        if not torch.cuda.is_available():
+            raise RuntimeError("rkstiff_amd needs a HIP device (torch.cuda.is_available() is False)")
+        h = ctypes.c_void_p()
+        Yc = np.ascontiguousarray(Y, dtype=np.float64)
+        _lib.check(lib.rks_hankel_create(int(nr), Yc.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                         float(rmax) ** 2 / jN, ctypes.byref(h)))
+        self._release()
+        self._lib, self._h = lib, h
+        self._nr, self._rmax = int(nr), float(rmax)
+        self._jN = jN
+        self._zeros = zeros
+        self._Y = torch.as_tensor(Y, dtype=torch.float64)
+        self.r, self.kr = r.to(self._device), kr.to(self._device)
+
+    def _release(self):
+        if self._h is not None and self._h.value:
+            try:
+                self._lib.rks_hankel_destroy(self._h)
+            except Exception:
+                pass
+        self._h = None
+
+    def __del__(self):
+        self._release()
+
+    def _scalefactor(self):
+        """S = rmax^2 / j_{N+1} (transforms.py:119-121)."""
+        return self._rmax**2 / self._jN
+
+    # -- accessors (transforms.py:127-152) -----------------------------------
+    def hankel_matrix(self):
+        return self._Y.clone()
+
+    def bessel_zeros(self):
+        return self._zeros.clone()
+
+    @property
+    def nr(self):
+        return self._nr
+
+    @nr.setter
+    def nr(self, nr):
+        if not isinstance(nr, (int, np.integer)) or nr < 4:
+            raise ValueError("nr must be an integer ≥ 4.")
+        self._setnr_setrmax(int(nr), self._rmax)
+
+    @property
+    def rmax(self):
+        return self._rmax
+
+    @rmax.setter
+    def rmax(self, rmax):
+        if rmax <= 0:
+            raise ValueError("rmax must be positive.")
+        self._setnr_setrmax(self._nr, float(rmax))
+
+    # -- transforms ------------------------------------------------------------
+    def _prep(self, v):
+        if not isinstance(v, torch.Tensor) or not v.is_cuda:
+            raise TypeError("HankelTransform works on device (HIP) tensors")
+        if v.shape[-1] != self._nr:
+            raise ValueError(f"expected {self._nr} values along the last axis, got {v.shape[-1]}")
+        if v.is_complex():
+            v = v.to(torch.complex128)
+        else:
+            v = v.to(torch.float64)
+        return v.contiguous()
+
+    def _apply(self, v, inverse):
+        v = self._prep(v)
+        out = torch.empty_like(v)
+        batch = v.numel() // self._nr
+        _lib.check(self._lib.rks_hankel_apply(self._h, int(inverse), ctypes.c_void_p(v.data_ptr()),
+                                              ctypes.c_void_p(out.data_ptr()), batch, int(v.is_complex()),
+                                              _stream()))
+        return out
+
+    def ht(self, f):
+        """Forward transform S Y f (transforms.py:172-203) along the last axis."""
+        return self._apply(f, False)
+
+    def iht(self, g):
+        """Inverse transform Y g / S (transforms.py:205-240) along the last axis."""
+        return self._apply(g, True)
+
+    def cubic_nl(self, gamma=1.0):
+        """nl_func(g) = 1j gamma ht(|f|^2 f), f = iht(g): the radial NLS term (demos/nls.ipynb
+        cell 8), on complex128 tensors of shape (..., nr)."""
+        def nl_func(g):
+            g = self._prep(g)
+            if not g.is_complex():
+                g = g.to(torch.complex128)
+            out = torch.empty_like(g)
+            _lib.check(self._lib.rks_hankel_cubic(self._h, float(gamma), ctypes.c_void_p(g.data_ptr()),
+                                                  ctypes.c_void_p(out.data_ptr()), g.numel() // self._nr,
+                                                  _stream()))
+            return out
+        return nl_func

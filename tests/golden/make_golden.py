@@ -35,6 +35,7 @@ def load_reference(root):
     sys.modules["rkstiff"] = pkg
     import rkstiff.etd35, rkstiff.etd34, rkstiff.etd4, rkstiff.etd5  # noqa: E401
     import rkstiff.if34, rkstiff.if45dp, rkstiff.if4, rkstiff.grids, rkstiff.models  # noqa: E401
+    import rkstiff.transforms  # noqa: F401
     return rkstiff
 
 
@@ -191,6 +192,42 @@ def case_cheb_allen_cahn(rk):
     print("I chebyshev allen-cahn done", flush=True)
 
 
+def case_hankel(rk):
+    """J: transforms.py HankelTransform (grid, zeros, matrix, ht of a Gaussian) and the radial
+    NLS of demos/nls.ipynb cells 6-12 (nr = 100, rmax = 5, L = -i kr^2, gamma = 2,
+    u0 = 2 exp(-r^2)) with ETD35 / ETD34 / IF34, epsilon 1e-5, tf = 0.8, store_freq = 10."""
+    from rkstiff.solveras import SolverConfig
+    out = {}
+    for nr, rmax in ((12, 2.0), (50, 2.0), (100, 5.0)):
+        ht = rk.transforms.HankelTransform(nr=nr, rmax=rmax)
+        key = f"{nr}_{rmax:g}"
+        out[f"{key}/r"], out[f"{key}/kr"] = ht.r, ht.kr
+        out[f"{key}/zeros"], out[f"{key}/Y"] = ht.bessel_zeros(), ht.hankel_matrix()
+        f = np.exp(-16.0 * ht.r**2) * (1.0 + 0.5j * ht.r)
+        out[f"{key}/f"], out[f"{key}/ht_f"], out[f"{key}/iht_f"] = f, ht.ht(f), ht.iht(f)
+    HT = rk.transforms.HankelTransform(nr=100, rmax=5)
+    L = -1j * HT.kr**2
+    gamma = 2
+
+    def nl_func(g):
+        f = HT.iht(g)
+        f2 = f.real**2 + f.imag**2
+        return 1j * gamma * HT.ht(f2 * f)
+
+    u0HT = HT.ht(2.0 * np.exp(-HT.r**2))
+    out["nls/L"], out["nls/u0"] = L, u0HT
+    classes = {"ETD35": rk.etd35.ETD35, "ETD34": rk.etd34.ETD34, "IF34": rk.if34.IF34}
+    for mname, cls in classes.items():
+        sv = logged(cls)(lin_op=L, nl_func=nl_func, config=SolverConfig(epsilon=1e-5))
+        uf = sv.evolve(u0HT, t0=0.0, tf=0.8, store_data=True, store_freq=10)
+        lh, ls = log_arrays(sv)
+        out[f"nls/{mname}/u_final"], out[f"nls/{mname}/t"] = uf, np.array(sv.t)
+        out[f"nls/{mname}/log_h"], out[f"nls/{mname}/log_s"] = lh, ls
+        out[f"nls/{mname}/nl_u0"] = nl_func(u0HT)
+    np.savez_compressed(os.path.join(HERE, "hankel.npz"), **out)
+    print("J hankel done", flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--reference", default="/root/reference")
@@ -206,6 +243,8 @@ def main():
             case_dense(rk)
         if "I" in only:
             case_cheb_allen_cahn(rk)
+        if "J" in only:
+            case_hankel(rk)
         return
     from rkstiff.solveras import SolverConfig
     from rkstiff.etd import ETDConfig
@@ -379,6 +418,7 @@ def main():
     case_diagonalized(rk)
     case_dense(rk)
     case_cheb_allen_cahn(rk)
+    case_hankel(rk)
 
     calib["cpu"] = os.uname().machine
     calib["numpy"] = np.__version__
