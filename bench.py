@@ -152,6 +152,10 @@ MODES = {  # name: (precision id, state element bytes, NL element bytes)
 }
 
 
+def progress(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 def sig(v, n=4):
     return float(f"{v:.{n}g}") if isinstance(v, float) else v
 
@@ -236,6 +240,7 @@ def run_modes(method, descs, nl, u0_host, tf, eps, modes, steps, warmup, floor=F
     res, fin = {}, {}
     for name in modes:
         res[name], fin[name] = measure(method, descs, nl, u0_host, tf, eps, name, steps, warmup)
+        progress(f"{method} {name}: {res[name]['value']:.1f} trajectories/s over {len(descs)}")
     for name in modes:
         e = np.zeros(1) if name == "f64" else rel_errors(fin[name], fin["f64"])
         res[name]["rel_l2_max"] = float(e.max())
@@ -313,6 +318,7 @@ def learned_workload(args):
         s.record(6000)
         run(s, ens, tf)
         datas.append(training_set(*s.records(), s.plan.cfg))
+        progress(f"learned: {name} training records {datas[-1].shape[0]}")
     data = torch.cat(datas)
     g = torch.Generator().manual_seed(0)
     data = data[torch.randperm(data.shape[0], generator=g)[: args.learned_records].to(data.device)].contiguous()
@@ -361,6 +367,7 @@ def learned_workload(args):
             tot[pol] = tot.get(pol, 0.0) + t_l
         s.set_proposer(None)
         out["families"][name] = fam
+        progress(f"learned: {name} evaluated")
     n_all = 2 * args.learned_count
     out["combined"] = {k + "_traj_per_s": sig(n_all / v) for k, v in tot.items()}
     return out
@@ -471,6 +478,7 @@ def main():
     }
     if not args.no_cpu:
         workers = args.cpu_workers or min(16, os.cpu_count() or 1)
+        progress(f"cpu baseline on {workers} cores")
         line["cpu_baseline"] = cpu_baseline(workers)
     print(json.dumps(line), flush=True)
 

@@ -93,66 +93,12 @@ __global__ void rks_f64_to_f32(const double* a, float* b, int n) {
 
 
 // explicit instantiations are in rks_k_<prec>_<nl>.hip
-extern template __global__ void rks_kernel<double, double, 4, NLK_ADVECT>(rks_launch);
-extern template __global__ void rks_kernel<double, double, 5, NLK_ADVECT>(rks_launch);
-extern template __global__ void rks_kernel<double, double, 6, NLK_ADVECT>(rks_launch);
-extern template __global__ void rks_kernel<double, double, 7, NLK_ADVECT>(rks_launch);
-extern template __global__ void rks_kernel<double, double, 8, NLK_ADVECT>(rks_launch);
-extern template __global__ void rks_kernel<double, double, 9, NLK_ADVECT>(rks_launch);
-extern template __global__ void rks_kernel<double, double, 10, NLK_ADVECT>(rks_launch);
-extern template __global__ void rks_kernel<double, double, 11, NLK_ADVECT>(rks_launch);
-extern template __global__ void rks_kernel<double, double, 12, NLK_ADVECT>(rks_launch);
-extern template __global__ void rks_kernel<double, double, 13, NLK_ADVECT>(rks_launch);
-extern template __global__ void rks_kernel<double, double, 4, NLK_CUBIC>(rks_launch);
-extern template __global__ void rks_kernel<double, double, 5, NLK_CUBIC>(rks_launch);
-extern template __global__ void rks_kernel<double, double, 6, NLK_CUBIC>(rks_launch);
-extern template __global__ void rks_kernel<double, double, 7, NLK_CUBIC>(rks_launch);
-extern template __global__ void rks_kernel<double, double, 8, NLK_CUBIC>(rks_launch);
-extern template __global__ void rks_kernel<double, double, 9, NLK_CUBIC>(rks_launch);
-extern template __global__ void rks_kernel<double, double, 10, NLK_CUBIC>(rks_launch);
-extern template __global__ void rks_kernel<double, double, 11, NLK_CUBIC>(rks_launch);
-extern template __global__ void rks_kernel<double, double, 12, NLK_CUBIC>(rks_launch);
-extern template __global__ void rks_kernel<double, double, 13, NLK_CUBIC>(rks_launch);
-extern template __global__ void rks_kernel<float, float, 4, NLK_ADVECT>(rks_launch);
-extern template __global__ void rks_kernel<float, float, 5, NLK_ADVECT>(rks_launch);
-extern template __global__ void rks_kernel<float, float, 6, NLK_ADVECT>(rks_launch);
-extern template __global__ void rks_kernel<float, float, 7, NLK_ADVECT>(rks_launch);
-extern template __global__ void rks_kernel<float, float, 8, NLK_ADVECT>(rks_launch);
-extern template __global__ void rks_kernel<float, float, 9, NLK_ADVECT>(rks_launch);
-extern template __global__ void rks_kernel<float, float, 10, NLK_ADVECT>(rks_launch);
-extern template __global__ void rks_kernel<float, float, 11, NLK_ADVECT>(rks_launch);
-extern template __global__ void rks_kernel<float, float, 12, NLK_ADVECT>(rks_launch);
-extern template __global__ void rks_kernel<float, float, 13, NLK_ADVECT>(rks_launch);
-extern template __global__ void rks_kernel<float, float, 4, NLK_CUBIC>(rks_launch);
-extern template __global__ void rks_kernel<float, float, 5, NLK_CUBIC>(rks_launch);
-extern template __global__ void rks_kernel<float, float, 6, NLK_CUBIC>(rks_launch);
-extern template __global__ void rks_kernel<float, float, 7, NLK_CUBIC>(rks_launch);
-extern template __global__ void rks_kernel<float, float, 8, NLK_CUBIC>(rks_launch);
-extern template __global__ void rks_kernel<float, float, 9, NLK_CUBIC>(rks_launch);
-extern template __global__ void rks_kernel<float, float, 10, NLK_CUBIC>(rks_launch);
-extern template __global__ void rks_kernel<float, float, 11, NLK_CUBIC>(rks_launch);
-extern template __global__ void rks_kernel<float, float, 12, NLK_CUBIC>(rks_launch);
-extern template __global__ void rks_kernel<float, float, 13, NLK_CUBIC>(rks_launch);
-extern template __global__ void rks_kernel<double, float, 4, NLK_ADVECT>(rks_launch);
-extern template __global__ void rks_kernel<double, float, 5, NLK_ADVECT>(rks_launch);
-extern template __global__ void rks_kernel<double, float, 6, NLK_ADVECT>(rks_launch);
-extern template __global__ void rks_kernel<double, float, 7, NLK_ADVECT>(rks_launch);
-extern template __global__ void rks_kernel<double, float, 8, NLK_ADVECT>(rks_launch);
-extern template __global__ void rks_kernel<double, float, 9, NLK_ADVECT>(rks_launch);
-extern template __global__ void rks_kernel<double, float, 10, NLK_ADVECT>(rks_launch);
-extern template __global__ void rks_kernel<double, float, 11, NLK_ADVECT>(rks_launch);
-extern template __global__ void rks_kernel<double, float, 12, NLK_ADVECT>(rks_launch);
-extern template __global__ void rks_kernel<double, float, 13, NLK_ADVECT>(rks_launch);
-extern template __global__ void rks_kernel<double, float, 4, NLK_CUBIC>(rks_launch);
-extern template __global__ void rks_kernel<double, float, 5, NLK_CUBIC>(rks_launch);
-extern template __global__ void rks_kernel<double, float, 6, NLK_CUBIC>(rks_launch);
-extern template __global__ void rks_kernel<double, float, 7, NLK_CUBIC>(rks_launch);
-extern template __global__ void rks_kernel<double, float, 8, NLK_CUBIC>(rks_launch);
-extern template __global__ void rks_kernel<double, float, 9, NLK_CUBIC>(rks_launch);
-extern template __global__ void rks_kernel<double, float, 10, NLK_CUBIC>(rks_launch);
-extern template __global__ void rks_kernel<double, float, 11, NLK_CUBIC>(rks_launch);
-extern template __global__ void rks_kernel<double, float, 12, NLK_CUBIC>(rks_launch);
-extern template __global__ void rks_kernel<double, float, 13, NLK_CUBIC>(rks_launch);
+RKS_KERNEL_SIZES(RKS_DECLARE, double, double, NLK_ADVECT)
+RKS_KERNEL_SIZES(RKS_DECLARE, double, double, NLK_CUBIC)
+RKS_KERNEL_SIZES(RKS_DECLARE, float, float, NLK_ADVECT)
+RKS_KERNEL_SIZES(RKS_DECLARE, float, float, NLK_CUBIC)
+RKS_KERNEL_SIZES(RKS_DECLARE, double, float, NLK_ADVECT)
+RKS_KERNEL_SIZES(RKS_DECLARE, double, float, NLK_CUBIC)
 
 // ===========================================================================
 // host side: plans and the C ABI
@@ -193,39 +139,52 @@ struct KernelEntry {
   int lds;
 };
 
-template <typename R, typename RN, int LOG2N, int NLK>
+template <typename R, typename RN, int LOG2N, int NLK, bool LE>
 KernelEntry entry_of() {
   KernelEntry e;
-  e.fn = (const void*)&rks_kernel<R, RN, LOG2N, NLK>;
+  e.fn = (const void*)&rks_kernel<R, RN, LOG2N, NLK, LE>;
   e.threads = rks_threads(LOG2N, NLK);
   e.lds = rks_lds_bytes<R, LOG2N>();
   return e;
 }
 
 template <typename R, typename RN, int NLK>
-bool lookup_nlk(int log2n, KernelEntry* e) {
+bool lookup_nlk(int log2n, bool learn, KernelEntry* e) {
+  if (learn) {
+    switch (log2n) {
+      case 7: *e = entry_of<R, RN, 7, NLK, true>(); return true;
+      case 8: *e = entry_of<R, RN, 8, NLK, true>(); return true;
+      case 9: *e = entry_of<R, RN, 9, NLK, true>(); return true;
+      case 10: *e = entry_of<R, RN, 10, NLK, true>(); return true;
+      case 11: *e = entry_of<R, RN, 11, NLK, true>(); return true;
+      case 12: *e = entry_of<R, RN, 12, NLK, true>(); return true;
+      case 13: *e = entry_of<R, RN, 13, NLK, true>(); return true;
+      default: return false;
+    }
+  }
   switch (log2n) {
-    case 4: *e = entry_of<R, RN, 4, NLK>(); return true;
-    case 5: *e = entry_of<R, RN, 5, NLK>(); return true;
-    case 6: *e = entry_of<R, RN, 6, NLK>(); return true;
-    case 7: *e = entry_of<R, RN, 7, NLK>(); return true;
-    case 8: *e = entry_of<R, RN, 8, NLK>(); return true;
-    case 9: *e = entry_of<R, RN, 9, NLK>(); return true;
-    case 10: *e = entry_of<R, RN, 10, NLK>(); return true;
-    case 11: *e = entry_of<R, RN, 11, NLK>(); return true;
-    case 12: *e = entry_of<R, RN, 12, NLK>(); return true;
-    case 13: *e = entry_of<R, RN, 13, NLK>(); return true;
+    case 4: *e = entry_of<R, RN, 4, NLK, false>(); return true;
+    case 5: *e = entry_of<R, RN, 5, NLK, false>(); return true;
+    case 6: *e = entry_of<R, RN, 6, NLK, false>(); return true;
+    case 7: *e = entry_of<R, RN, 7, NLK, false>(); return true;
+    case 8: *e = entry_of<R, RN, 8, NLK, false>(); return true;
+    case 9: *e = entry_of<R, RN, 9, NLK, false>(); return true;
+    case 10: *e = entry_of<R, RN, 10, NLK, false>(); return true;
+    case 11: *e = entry_of<R, RN, 11, NLK, false>(); return true;
+    case 12: *e = entry_of<R, RN, 12, NLK, false>(); return true;
+    case 13: *e = entry_of<R, RN, 13, NLK, false>(); return true;
     default: return false;
   }
 }
 
-bool lookup(int prec, int nlk, int log2n, KernelEntry* e) {
-  if (prec == RKS_F64) return nlk == NLK_ADVECT ? lookup_nlk<double, double, NLK_ADVECT>(log2n, e)
-                                                : lookup_nlk<double, double, NLK_CUBIC>(log2n, e);
-  if (prec == RKS_MIXED) return nlk == NLK_ADVECT ? lookup_nlk<double, float, NLK_ADVECT>(log2n, e)
-                                                  : lookup_nlk<double, float, NLK_CUBIC>(log2n, e);
-  return nlk == NLK_ADVECT ? lookup_nlk<float, float, NLK_ADVECT>(log2n, e)
-                           : lookup_nlk<float, float, NLK_CUBIC>(log2n, e);
+// learn: the proposer variant (a proposer or training records are attached)
+bool lookup(int prec, int nlk, int log2n, bool learn, KernelEntry* e) {
+  if (prec == RKS_F64) return nlk == NLK_ADVECT ? lookup_nlk<double, double, NLK_ADVECT>(log2n, learn, e)
+                                                : lookup_nlk<double, double, NLK_CUBIC>(log2n, learn, e);
+  if (prec == RKS_MIXED) return nlk == NLK_ADVECT ? lookup_nlk<double, float, NLK_ADVECT>(log2n, learn, e)
+                                                  : lookup_nlk<double, float, NLK_CUBIC>(log2n, learn, e);
+  return nlk == NLK_ADVECT ? lookup_nlk<float, float, NLK_ADVECT>(log2n, learn, e)
+                           : lookup_nlk<float, float, NLK_CUBIC>(log2n, learn, e);
 }
 
 int ilog2(int n) {
@@ -486,11 +445,14 @@ int rks_plan_create(int method, int nl_kind, int precision, const rks_config* cf
       p->bucket_count.push_back((int)byl[l].size());
       for (int i : byl[l]) p->order.push_back(i);
       KernelEntry e;
-      if (!lookup(precision, nl_kind, l, &e)) {
+      if (!lookup(precision, nl_kind, l, false, &e)) {
         rks_plan_destroy(p);
         return fail(RKS_ERR_INVALID, "no kernel for this size");
       }
       HIPCHK(hipFuncSetAttribute(e.fn, hipFuncAttributeMaxDynamicSharedMemorySize, e.lds));
+      KernelEntry el;
+      if (lookup(precision, nl_kind, l, true, &el))
+        HIPCHK(hipFuncSetAttribute(el.fn, hipFuncAttributeMaxDynamicSharedMemorySize, el.lds));
       const int N = 1 << l;
       HIPCHK(hipMalloc(&p->d_tw[l], (size_t)N * p->esize));
       if (precision != RKS_F32)
@@ -626,7 +588,9 @@ static int launch_all(rks_plan* p, int mode, double h_mode, hipStream_t stream) 
   HIPCHK(hipGetLastError());
   for (size_t b = 0; b < p->bucket_log2.size(); ++b) {
     KernelEntry e;
-    lookup(p->prec, p->nlk, p->bucket_log2[b], &e);
+    const bool learn = p->mlp.on || p->d_rec != nullptr;
+    if (!lookup(p->prec, p->nlk, p->bucket_log2[b], learn, &e))
+      return fail(RKS_ERR_INVALID, "no proposer kernel for this grid size");
     L.order = p->d_order + p->bucket_start[b];
     L.count = p->bucket_count[b];
     L.queue = p->d_queue + b;

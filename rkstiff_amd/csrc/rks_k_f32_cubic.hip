@@ -1,13 +1,5 @@
-// Persistent stepping kernels: state/FFT float, NL and coefficient storage float, cubic nonlinearity, n = 2^4 .. 2^13.
+// Persistent stepping kernels: state/FFT float, NL and coefficient storage float, cubic nonlinearity, n = 2^4 .. 2^13
+// (and the proposer variants, n = 2^7 .. 2^13).
 #include "rks_kernels.h"
 
-template __global__ void rks_kernel<float, float, 4, NLK_CUBIC>(rks_launch);
-template __global__ void rks_kernel<float, float, 5, NLK_CUBIC>(rks_launch);
-template __global__ void rks_kernel<float, float, 6, NLK_CUBIC>(rks_launch);
-template __global__ void rks_kernel<float, float, 7, NLK_CUBIC>(rks_launch);
-template __global__ void rks_kernel<float, float, 8, NLK_CUBIC>(rks_launch);
-template __global__ void rks_kernel<float, float, 9, NLK_CUBIC>(rks_launch);
-template __global__ void rks_kernel<float, float, 10, NLK_CUBIC>(rks_launch);
-template __global__ void rks_kernel<float, float, 11, NLK_CUBIC>(rks_launch);
-template __global__ void rks_kernel<float, float, 12, NLK_CUBIC>(rks_launch);
-template __global__ void rks_kernel<float, float, 13, NLK_CUBIC>(rks_launch);
+RKS_KERNEL_SIZES(RKS_INSTANTIATE, float, float, NLK_CUBIC)

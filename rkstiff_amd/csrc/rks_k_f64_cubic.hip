@@ -1,13 +1,5 @@
-// Persistent stepping kernels: state/FFT double, NL and coefficient storage double, cubic nonlinearity, n = 2^4 .. 2^13.
+// Persistent stepping kernels: state/FFT double, NL and coefficient storage double, cubic nonlinearity, n = 2^4 .. 2^13
+// (and the proposer variants, n = 2^7 .. 2^13).
 #include "rks_kernels.h"
 
-template __global__ void rks_kernel<double, double, 4, NLK_CUBIC>(rks_launch);
-template __global__ void rks_kernel<double, double, 5, NLK_CUBIC>(rks_launch);
-template __global__ void rks_kernel<double, double, 6, NLK_CUBIC>(rks_launch);
-template __global__ void rks_kernel<double, double, 7, NLK_CUBIC>(rks_launch);
-template __global__ void rks_kernel<double, double, 8, NLK_CUBIC>(rks_launch);
-template __global__ void rks_kernel<double, double, 9, NLK_CUBIC>(rks_launch);
-template __global__ void rks_kernel<double, double, 10, NLK_CUBIC>(rks_launch);
-template __global__ void rks_kernel<double, double, 11, NLK_CUBIC>(rks_launch);
-template __global__ void rks_kernel<double, double, 12, NLK_CUBIC>(rks_launch);
-template __global__ void rks_kernel<double, double, 13, NLK_CUBIC>(rks_launch);
+RKS_KERNEL_SIZES(RKS_INSTANTIATE, double, double, NLK_CUBIC)

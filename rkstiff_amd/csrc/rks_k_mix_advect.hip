@@ -1,13 +1,5 @@
-// Persistent stepping kernels: state/FFT double, NL and coefficient storage float, advect nonlinearity, n = 2^4 .. 2^13.
+// Persistent stepping kernels: state/FFT double, NL and coefficient storage float, advect nonlinearity, n = 2^4 .. 2^13
+// (and the proposer variants, n = 2^7 .. 2^13).
 #include "rks_kernels.h"
 
-template __global__ void rks_kernel<double, float, 4, NLK_ADVECT>(rks_launch);
-template __global__ void rks_kernel<double, float, 5, NLK_ADVECT>(rks_launch);
-template __global__ void rks_kernel<double, float, 6, NLK_ADVECT>(rks_launch);
-template __global__ void rks_kernel<double, float, 7, NLK_ADVECT>(rks_launch);
-template __global__ void rks_kernel<double, float, 8, NLK_ADVECT>(rks_launch);
-template __global__ void rks_kernel<double, float, 9, NLK_ADVECT>(rks_launch);
-template __global__ void rks_kernel<double, float, 10, NLK_ADVECT>(rks_launch);
-template __global__ void rks_kernel<double, float, 11, NLK_ADVECT>(rks_launch);
-template __global__ void rks_kernel<double, float, 12, NLK_ADVECT>(rks_launch);
-template __global__ void rks_kernel<double, float, 13, NLK_ADVECT>(rks_launch);
+RKS_KERNEL_SIZES(RKS_INSTANTIATE, double, float, NLK_ADVECT)

@@ -682,8 +682,7 @@ __device__ __forceinline__ void eval_nl(GetK getk, cx<R>* buf, const cx<R>* twl,
 template <typename R, int LOG2N, int T, int NLK>
 __device__ __attribute__((noinline)) double learned_after_accept(const rks_launch& P, const rks_ctrl& ct, rks_inst* in,
                                        const cx<R>* ucand, const cx<R>* buf, bool fused, double h_used,
-                                       double s_acc, double h_suggest, float* bins, float* scr,
-                                       int& rec_count, int& rec_pending) {
+                                       double s_acc, double h_suggest, float* bins, float* scr) {
   static_assert(T >= RKS_MLP_NBIN && T % 64 == 0, "one thread per spectral input, whole waves");
   constexpr int N = 1 << LOG2N;
   constexpr int M = (NLK == NLK_ADVECT) ? N / 2 + 1 : N;
@@ -715,15 +714,17 @@ __device__ __attribute__((noinline)) double learned_after_accept(const rks_launc
     bins[RKS_MLP_NBIN + 1] = (float)log(s_acc < RKS_MLP_S_CAP ? s_acc : RKS_MLP_S_CAP);
   }
   __syncthreads();
+  const int rec_count = in->rec_count;
   if (in->rec != nullptr && rec_count < in->rec_cap) {
     float* r = in->rec + (size_t)rec_count * RKS_MLP_REC;
     for (int j = t; j < RKS_MLP_NIN; j += T) r[j] = bins[j];
+    __syncthreads();  // every thread has read the counters before they change
     if (t == 0) {
       r[RKS_MLP_NIN] = __builtin_nanf("");
       r[RKS_MLP_NIN + 1] = 0.f;
+      in->rec_pending = rec_count;
+      in->rec_count = rec_count + 1;
     }
-    rec_pending = rec_count;
-    ++rec_count;
   }
   if (P.mlp.on) {
     rks_mlp_standardize<T>(P.mlp, bins, scr);
@@ -750,7 +751,7 @@ __device__ __attribute__((noinline)) double learned_after_accept(const rks_launc
 // ---------------------------------------------------------------------------
 // one instance: evolve / step / nl / coefficients
 // ---------------------------------------------------------------------------
-template <typename R, typename RN, int LOG2N, int T, int NLK>
+template <typename R, typename RN, int LOG2N, int T, int NLK, bool LEARN>
 __device__ void run_instance(const rks_launch& P, int idx, cx<R>* buf, const cx<R>* twl, double* red,
                              int* lds_count, float* bins, float* scr) {
   using C = cx<R>;
@@ -805,8 +806,6 @@ __device__ void run_instance(const rks_launch& P, int idx, cx<R>* buf, const cx<
   int rejects = in->rejects, status = in->status, ubuf = in->ubuf;
   int log_count = in->log_count, snap_count = in->snap_count;
   int nl_calls = in->nl_calls, coef_updates = in->coef_updates;
-  int rec_count = in->rec_count, rec_pending = in->rec_pending;
-  double h_prop = in->h_prop;
   // slot map packed 4 bits per slot (no dynamically indexed register array)
   unsigned smap = 0;
 #pragma unroll
@@ -828,12 +827,19 @@ __device__ void run_instance(const rks_launch& P, int idx, cx<R>* buf, const cx<
     steps = attempts = rejects = 0;
     log_count = snap_count = 0;
     nl_calls = coef_updates = 0;
-    rec_count = 0;
-    rec_pending = -1;
     ubuf = -1;
     status = RKS_ST_RUNNING;
     if (tb.adaptive && tc + h > tf) h = tf - tc;  // solveras.py:620-621
-    h_prop = h;
+    if (LEARN && in->rec != nullptr) {
+      // record counters live in the instance record: the stepping loop does not carry them
+      __syncthreads();
+      if (t == 0) {
+        in->rec_count = 0;
+        in->rec_pending = -1;
+        in->h_prop = h;
+      }
+      __syncthreads();
+    }
     if (store_freq > 0) {
       for (int i = 0; i < MPT; ++i) {
         const int m = t + i * T;
@@ -869,7 +875,8 @@ __device__ void run_instance(const rks_launch& P, int idx, cx<R>* buf, const cx<
     const C* u = evolve ? cur_u() : uin;
     C* cand = evolve ? (ubuf == 0 ? cand1 : cand0) : (C*)io.u_out;
     int numloops = 0;
-    double h_suggest = h, s_acc = 0.0;
+    double h_suggest = h;
+    double s_acc;  // s of the accepted attempt (set where the attempt loop accepts)
     // ---------------- attempt loop (solveras.py:384-410) ----------------
     for (;;) {
       if (h != h_coeff) {
@@ -999,15 +1006,17 @@ __device__ void run_instance(const rks_launch& P, int idx, cx<R>* buf, const cx<
         in->log[2 * log_count + 1] = sv;
       }
       ++log_count;
-      if (evolve && rec_pending >= 0 && in->rec != nullptr) {
+      if (LEARN && evolve && in->rec != nullptr) {
         // target of the record left by the last accept: ln(h s / h_accepted)
-        if (t == 0) {
-          float* r = in->rec + (size_t)rec_pending * RKS_MLP_REC;
-          const bool ok = (h == h_prop) && !(isinf(sv) || isnan(sv)) && sv < 1e300;
+        const int pend = in->rec_pending;
+        __syncthreads();  // every thread has read rec_pending before thread 0 clears it
+        if (pend >= 0 && t == 0) {
+          float* r = in->rec + (size_t)pend * RKS_MLP_REC;
+          const bool ok = (h == in->h_prop) && !(isinf(sv) || isnan(sv)) && sv < 1e300;
           r[RKS_MLP_NIN] = (float)log(h * sv / h_used);
           r[RKS_MLP_NIN + 1] = ok ? 1.f : 0.f;
+          in->rec_pending = -1;
         }
-        rec_pending = -1;
       }
       if (isinf(sv) || isnan(sv) || sv < 1.0) {
         // reject (solveras.py:490-506)
@@ -1059,14 +1068,17 @@ __device__ void run_instance(const rks_launch& P, int idx, cx<R>* buf, const cx<
       status = RKS_ST_MINSTEP;
       break;
     }
-    if constexpr (T >= 64) {
+    if constexpr (LEARN && T >= 64) {
       if (tb.adaptive && tc < tf && (P.mlp.on || in->rec != nullptr)) {
         h_suggest = learned_after_accept<R, LOG2N, T, NLK>(P, ct, in, cur_u(), buf, fused, h_used, s_acc,
-                                                           h_suggest, bins, scr, rec_count, rec_pending);
+                                                           h_suggest, bins, scr);
         h_next = h_suggest;
+        if (in->rec != nullptr) {
+          __syncthreads();
+          if (t == 0) in->h_prop = h_suggest;
+        }
       }
     }
-    h_prop = h_suggest;
     if (tb.adaptive) h = (tc + h_suggest > tf) ? tf - tc : h_suggest;  // solveras.py:638-641
     if (store_freq > 0 && (steps % store_freq) == 0) {
       C* dst = (C*)in->snap + (size_t)snap_count * M;
@@ -1106,9 +1118,6 @@ __device__ void run_instance(const rks_launch& P, int idx, cx<R>* buf, const cx<
     in->snap_count = snap_count;
     in->nl_calls = nl_calls;
     in->coef_updates = coef_updates;
-    in->rec_count = rec_count;
-    in->rec_pending = rec_pending;
-    in->h_prop = h_prop;
 #pragma unroll
     for (int s = 0; s < RKS_MAX_SLOTS; ++s) in->slot_map[s] = (int)((smap >> (4 * s)) & 15u);
   }
@@ -1119,7 +1128,10 @@ __device__ void run_instance(const rks_launch& P, int idx, cx<R>* buf, const cx<
 // coefficient code spills instead of lowering occupancy.
 __host__ __device__ constexpr int rks_min_waves(int log2n) { return log2n >= 13 ? 2 : 3; }
 
-template <typename R, typename RN, int LOG2N, int NLK>
+// LEARN: the variant with the learned step-size proposer and its training
+// records; the plain variant carries none of that code, so the stepping loop
+// keeps its register allocation.
+template <typename R, typename RN, int LOG2N, int NLK, bool LEARN>
 __global__ __launch_bounds__(rks_threads(LOG2N, NLK), rks_min_waves(LOG2N)) void rks_kernel(rks_launch P) {
   constexpr int T = rks_threads(LOG2N, NLK);
   extern __shared__ __align__(16) unsigned char rks_smem[];
@@ -1130,8 +1142,8 @@ __global__ __launch_bounds__(rks_threads(LOG2N, NLK), rks_min_waves(LOG2N)) void
   // FFT buffer, idle between attempts, when it is large enough)
   constexpr bool kScrInBuf =
       rksfft::padded_size(1 << LOG2N) * sizeof(cx<R>) >= RKS_MLP_LDS_FLOATS * sizeof(float);
-  __shared__ float s_bins[2 * RKS_MLP_NBIN];
-  __shared__ float s_scr[kScrInBuf ? 1 : RKS_MLP_LDS_FLOATS];
+  __shared__ float s_bins[LEARN ? 2 * RKS_MLP_NBIN : 1];
+  __shared__ float s_scr[(LEARN && !kScrInBuf) ? RKS_MLP_LDS_FLOATS : 1];
   cx<R>* buf = (cx<R>*)rks_smem;
   cx<R>* twl = buf + rksfft::padded_size(1 << LOG2N);
   float* scr = kScrInBuf ? (float*)rks_smem : s_scr;
@@ -1142,8 +1154,20 @@ __global__ __launch_bounds__(rks_threads(LOG2N, NLK), rks_min_waves(LOG2N)) void
     const int item = s_item;
     __syncthreads();
     if (item >= P.count) break;
-    run_instance<R, RN, LOG2N, T, NLK>(P, P.order[item], buf, twl, red, &s_count, s_bins, scr);
+    run_instance<R, RN, LOG2N, T, NLK, LEARN>(P, P.order[item], buf, twl, red, &s_count, s_bins, scr);
     __syncthreads();
   }
 }
 
+
+// Explicit instantiations of one precision x nonlinearity (rks_k_*.hip): every
+// size, and the proposer variant for the sizes whose workgroups can be whole
+// waves (n >= 128; the host refuses smaller rfft grids).
+#define RKS_KERNEL_SIZES(X, R, RN, NLK)                                                              \
+  X(R, RN, 4, NLK, false) X(R, RN, 5, NLK, false) X(R, RN, 6, NLK, false) X(R, RN, 7, NLK, false)   \
+  X(R, RN, 8, NLK, false) X(R, RN, 9, NLK, false) X(R, RN, 10, NLK, false) X(R, RN, 11, NLK, false) \
+  X(R, RN, 12, NLK, false) X(R, RN, 13, NLK, false) X(R, RN, 7, NLK, true) X(R, RN, 8, NLK, true)   \
+  X(R, RN, 9, NLK, true) X(R, RN, 10, NLK, true) X(R, RN, 11, NLK, true) X(R, RN, 12, NLK, true)    \
+  X(R, RN, 13, NLK, true)
+#define RKS_INSTANTIATE(R, RN, L, K, LE) template __global__ void rks_kernel<R, RN, L, K, LE>(rks_launch);
+#define RKS_DECLARE(R, RN, L, K, LE) extern template __global__ void rks_kernel<R, RN, L, K, LE>(rks_launch);
