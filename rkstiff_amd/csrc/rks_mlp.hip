@@ -9,9 +9,11 @@
 //               (forward), x GELU'(Z) (backward through a hidden layer), and
 //               plain FP32 (weight gradients)
 //   head        output layer, loss and its gradient, dZ of the last hidden layer
-//   colsum      bias gradients (deterministic column sums, no atomics)
+//   colsum      bias and head gradients (two-stage fixed-order column sums,
+//               no atomics); weight gradients are split over 8 K slices and
+//               the slices summed in order
 //   adam        Adam on the FP32 master parameters, refreshes the BF16 copies
-// One training step is 14 launches; nothing is allocated per step.
+// One training step is about 25 launches; nothing is allocated per step.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -51,8 +53,9 @@ struct GemmArgs {
   long sam, sak;
   const uint16_t* B;  // element (k, n) at B[k * sbk + n * sbn]
   long sbk, sbn;
-  float* C;           // EPI_F32: C[m * ldc + n]
+  float* C;           // EPI_F32: C[m * ldc + n] (+ z * cstride for K slice z of a split-K launch)
   int ldc;
+  long cstride;
   const float* bias;  // EPI_BIAS_GELU: Z[m * ldo + n] = acc + bias[n], O = bf16(gelu(Z))
   float* Z;
   uint16_t* O;        // EPI_GELU_BWD: O[m * ldo + n] = bf16(acc * gelu'(Zin[m * ldo + n]))
@@ -78,7 +81,9 @@ __global__ __launch_bounds__(256) void mlp_gemm(GemmArgs g) {
     for (int j = 0; j < 2; ++j) acc[i][j] = rks_f32x4{0.f, 0.f, 0.f, 0.f};
   const bool a_kfast = (g.sak == 1);
   const bool b_nfast = (g.sbn == 1);
-  for (int k0 = 0; k0 < g.K; k0 += GK) {
+  const int kper = g.K / (int)gridDim.z;  // split-K slice (gridDim.z = 1 otherwise)
+  const int kbeg = (int)blockIdx.z * kper;
+  for (int k0 = kbeg; k0 < kbeg + kper; k0 += GK) {
 #pragma unroll
     for (int e = 0; e < GT * GK / 256; ++e) {
       const int idx = t + 256 * e;
@@ -126,7 +131,7 @@ __global__ __launch_bounds__(256) void mlp_gemm(GemmArgs g) {
         const int m = m0 + wm + i * 16 + 4 * kq + r;
         const float v = acc[i][j][r];
         if constexpr (EPI == EPI_F32) {
-          g.C[(long)m * g.ldc + n] = v;
+          g.C[(long)blockIdx.z * g.cstride + (long)m * g.ldc + n] = v;
         } else if constexpr (EPI == EPI_BIAS_GELU) {
           const float z = v + g.bias[n];
           g.Z[(long)m * g.ldo + n] = z;
@@ -202,32 +207,44 @@ __global__ __launch_bounds__(256) void mlp_head(int B, int nvalid, const uint16_
   }
 }
 
-// out[n] = sum_b D[b][n] (BF16 D), one thread per column, fixed order.
-__global__ void mlp_colsum(int B, int N, const uint16_t* __restrict__ D, float* __restrict__ out) {
-  const int n = blockIdx.x * blockDim.x + threadIdx.x;
-  if (n >= N) return;
-  float s = 0.f;
-  for (int b = 0; b < B; ++b) s += rks_bf2f(D[(long)b * N + n]);
-  out[n] = s;
+// Column sums in two fixed-order stages (deterministic, no atomics).
+// Stage 1: block (x, c) sums rows of chunk c (B / CS rows) for 64 columns;
+// 4 row lanes per column, combined in LDS in lane order.
+//   MODE 0: part[c][n] = sum_b D[b][n]              (BF16 D, bias gradients)
+//   MODE 1: part[c][n] = sum_b w[b] D[b][n]         (head weight gradient)
+//   MODE 2: part[c][0] = sum_b w[b], part[c][1] = sum_b w2[b]   (head bias, loss)
+constexpr int CS = 16;     // row chunks of the column sums
+constexpr int KSPLIT = 8;  // K slices of a weight-gradient GEMM (K = batch)
+template <int MODE>
+__global__ __launch_bounds__(256) void mlp_colsum(int B, int N, const uint16_t* __restrict__ D,
+                                                  const float* __restrict__ w, const float* __restrict__ w2,
+                                                  float* __restrict__ part) {
+  __shared__ float red[4][64];
+  const int col = threadIdx.x % 64, lane = threadIdx.x / 64;
+  const int n = blockIdx.x * 64 + col, c = blockIdx.y;
+  const int rows = B / CS;
+  float acc = 0.f;
+  if (n < N) {
+    for (int r = lane; r < rows; r += 4) {
+      const long b = (long)c * rows + r;
+      if (MODE == 0) acc += rks_bf2f(D[b * N + n]);
+      else if (MODE == 1) acc += w[b] * rks_bf2f(D[b * N + n]);
+      else acc += (n == 0 ? w[b] : w2[b]);
+    }
+  }
+  red[lane][col] = acc;
+  __syncthreads();
+  if (lane == 0 && n < N) part[(long)c * N + n] = ((red[0][col] + red[1][col]) + red[2][col]) + red[3][col];
 }
 
-// Head gradients: gw4[k] = sum_b g_b A3[b][k]; gb4 = sum_b g_b; mean loss.
-__global__ void mlp_head_grad(int B, const float* __restrict__ g, const uint16_t* __restrict__ A3,
-                              const float* __restrict__ lossv, int nvalid, float* __restrict__ gw4,
-                              float* __restrict__ gb4, float* __restrict__ loss_out) {
-  const int k = threadIdx.x;
+// Stage 2: out[n] = scale * sum_c part[c * stride + n] (c in order).
+__global__ void mlp_sum_parts(int S, long N, long stride, const float* __restrict__ part, float* __restrict__ out,
+                              float scale) {
+  const long n = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= N) return;
   float s = 0.f;
-  for (int b = 0; b < B; ++b) s += g[b] * rks_bf2f(A3[(long)b * RKS_MLP_HID + k]);
-  gw4[k] = s;
-  if (k == 0) {
-    float sg = 0.f, sl = 0.f;
-    for (int b = 0; b < B; ++b) {
-      sg += g[b];
-      sl += lossv[b];
-    }
-    *gb4 = sg;
-    if (loss_out) *loss_out = sl / (float)nvalid;
-  }
+  for (int c = 0; c < S; ++c) s += part[(long)c * stride + n];
+  out[n] = s * scale;
 }
 
 // Index of flat parameter i in the BF16 weight copy (-1 for biases).
@@ -296,6 +313,7 @@ struct rks_mlp {
   uint16_t *X = nullptr, *A1 = nullptr, *A2 = nullptr, *A3 = nullptr, *D1 = nullptr, *D2 = nullptr, *D3 = nullptr;
   float *Z1 = nullptr, *Z2 = nullptr, *Z3 = nullptr, *y = nullptr, *yh = nullptr, *g = nullptr, *lv = nullptr;
   int* d_idx = nullptr;
+  float* part = nullptr;  // split-K slices of a weight gradient / column-sum chunks
 };
 
 namespace {
@@ -306,10 +324,12 @@ int mfail(int code, const std::string& msg) {
 }
 
 void free_ws(rks_mlp* n) {
-  void* ps[] = {n->X, n->A1, n->A2, n->A3, n->D1, n->D2, n->D3, n->Z1, n->Z2, n->Z3, n->y, n->yh, n->g, n->lv, n->d_idx};
+  void* ps[] = {n->X, n->A1, n->A2, n->A3, n->D1, n->D2, n->D3, n->Z1, n->Z2, n->Z3, n->y, n->yh, n->g, n->lv,
+                n->d_idx, n->part};
   for (void* p : ps) (void)hipFree(p);
   n->X = n->A1 = n->A2 = n->A3 = n->D1 = n->D2 = n->D3 = nullptr;
   n->Z1 = n->Z2 = n->Z3 = n->y = n->yh = n->g = n->lv = nullptr;
+  n->part = nullptr;
   n->d_idx = nullptr;
   n->cap = 0;
 }
@@ -332,14 +352,15 @@ int ensure_ws(rks_mlp* n, int B) {
   MLPCHK(hipMalloc(&n->yh, sizeof(float) * B));
   MLPCHK(hipMalloc(&n->g, sizeof(float) * B));
   MLPCHK(hipMalloc(&n->lv, sizeof(float) * B));
+  MLPCHK(hipMalloc(&n->part, sizeof(float) * KSPLIT * RKS_MLP_HID * RKS_MLP_HID));
   n->cap = B;
   return RKS_OK;
 }
 
 template <int EPI>
-int gemm(const GemmArgs& a, hipStream_t s) {
-  if (a.M % GT || a.K % GK || a.N % 16) return mfail(RKS_ERR_INVALID, "gemm shape");
-  dim3 grid((a.N + GT - 1) / GT, a.M / GT);
+int gemm(const GemmArgs& a, hipStream_t s, int ksplit = 1) {
+  if (a.M % GT || a.K % (GK * ksplit) || a.N % 16) return mfail(RKS_ERR_INVALID, "gemm shape");
+  dim3 grid((a.N + GT - 1) / GT, a.M / GT, ksplit);
   hipLaunchKernelGGL(mlp_gemm<EPI>, grid, dim3(256), 0, s, a);
   MLPCHK(hipGetLastError());
   return RKS_OK;
@@ -389,8 +410,17 @@ int backward_batch(rks_mlp* n, int B, int nvalid, float* loss_out, hipStream_t s
   const uint16_t* w3 = w2 + RKS_MLP_HID * RKS_MLP_HID;
   float* G = n->d_grad;
   int rc;
-  hipLaunchKernelGGL(mlp_head_grad, dim3(1), dim3(RKS_MLP_HID), 0, s, B, n->g, n->A3, n->lv, nvalid,
-                     G + RKS_MLP_OFF_W4, G + RKS_MLP_OFF_B4, loss_out);
+  // head: gw4 = sum_b g_b A3[b], gb4 = sum_b g_b, mean loss
+  hipLaunchKernelGGL(mlp_colsum<1>, dim3(RKS_MLP_HID / 64, CS), dim3(256), 0, s, B, RKS_MLP_HID, n->A3, n->g,
+                     (const float*)nullptr, n->part);
+  hipLaunchKernelGGL(mlp_sum_parts, dim3(1), dim3(RKS_MLP_HID), 0, s, CS, (long)RKS_MLP_HID, (long)RKS_MLP_HID,
+                     n->part, G + RKS_MLP_OFF_W4, 1.f);
+  hipLaunchKernelGGL(mlp_colsum<2>, dim3(1, CS), dim3(256), 0, s, B, 2, (const uint16_t*)nullptr, n->g, n->lv,
+                     n->part);
+  hipLaunchKernelGGL(mlp_sum_parts, dim3(1), dim3(64), 0, s, CS, 1L, 2L, n->part, G + RKS_MLP_OFF_B4, 1.f);
+  if (loss_out)
+    hipLaunchKernelGGL(mlp_sum_parts, dim3(1), dim3(64), 0, s, CS, 1L, 2L, n->part + 1, loss_out,
+                       1.f / (float)nvalid);
   MLPCHK(hipGetLastError());
   const uint16_t* acts[3] = {n->X, n->A1, n->A2};
   const int kin[3] = {RKS_MLP_K0, RKS_MLP_HID, RKS_MLP_HID};
@@ -411,10 +441,19 @@ int backward_batch(rks_mlp* n, int B, int nvalid, float* loss_out, hipStream_t s
     a.B = acts[l];
     a.sbk = kin[l];
     a.sbn = 1;
-    a.C = G + offW[l];
+    a.C = n->part;  // ks slices, summed in order below
     a.ldc = kin[l];
-    if ((rc = gemm<EPI_F32>(a, s))) return rc;
-    hipLaunchKernelGGL(mlp_colsum, dim3(RKS_MLP_HID / 64), dim3(64), 0, s, B, RKS_MLP_HID, dz[l], G + offB[l]);
+    a.cstride = (long)RKS_MLP_HID * kin[l];
+    int ks = KSPLIT;
+    while (ks > 1 && B % (GK * ks)) ks /= 2;
+    if ((rc = gemm<EPI_F32>(a, s, ks))) return rc;
+    const long nw = (long)RKS_MLP_HID * kin[l];
+    hipLaunchKernelGGL(mlp_sum_parts, dim3((unsigned)((nw + 255) / 256)), dim3(256), 0, s, ks, nw, nw, n->part,
+                       G + offW[l], 1.f);
+    hipLaunchKernelGGL(mlp_colsum<0>, dim3(RKS_MLP_HID / 64, CS), dim3(256), 0, s, B, RKS_MLP_HID, dz[l],
+                       (const float*)nullptr, (const float*)nullptr, n->part);
+    hipLaunchKernelGGL(mlp_sum_parts, dim3(1), dim3(RKS_MLP_HID), 0, s, CS, (long)RKS_MLP_HID, (long)RKS_MLP_HID,
+                       n->part, G + offB[l], 1.f);
     MLPCHK(hipGetLastError());
     if (l > 0) {
       // dZ_{l-1} = (dZ_l W_l) * gelu'(Z_{l-1})
