@@ -33,6 +33,11 @@ Also measured and reported under "workloads" (one timed pass each):
                 (trajectories/s, attempts, rejections, coefficient
                 regenerations, error vs FP64 with the step-sequence floor)
 
+Every measured plan first picks its launch shape (rkstiff_amd.autotune: size
+buckets sequential or overlapped on private streams, resident workgroups per
+CU) from pilot evolves to tf / 20; the line reports the shape and the pilot
+times.  Tuning is plan set-up and stays outside the timed region.
+
 Reported beside the value:
   * the plain per-instance FP64 drop-in (one ETD35.evolve per trajectory),
   * the HBM roofline of the stepping kernel: algorithmic bytes per launch
@@ -198,10 +203,13 @@ def measure(method, descs, nl, u0_host, tf, eps, mode, steps, warmup):
     prec, e_u, e_nl = MODES[mode]
     dt = torch.complex64 if mode == "f32" else torch.complex128
     B = len(descs)
+    from rkstiff_amd.autotune import tune_launch
     plan = Plan(method, nl, getattr(_lib, prec), SolverConfig(epsilon=eps), None, descs)
     u0 = [torch.as_tensor(u, dtype=dt, device="cuda") for u in u0_host]
     uf = [torch.empty_like(u) for u in u0]
     t0s, tfs, hs = [0.0] * B, [tf] * B, [tf / 100.0] * B
+    # launch shape from pilot evolves to tf / 20 (plan set-up, outside the timed region)
+    tuned = tune_launch(plan, u0, 0.0, tf / 20.0, tf / 2000.0)
     stream = torch.cuda.current_stream()
     for _ in range(warmup):
         plan.evolve(u0, uf, t0s, tfs, hs)
@@ -224,7 +232,9 @@ def measure(method, descs, nl, u0_host, tf, eps, mode, steps, warmup):
     k_ms = float(np.mean(kernel_ms))
     out = {"value": B * steps / wall, "ms_per_step": wall / steps * 1e3, "kernel_ms": k_ms,
            "achieved_GBs": alg / (k_ms * 1e-3) / 1e9, "attempts": float(np.mean([s["attempts"] for s in st])),
-           "ok": all(s["status"] == 0 for s in st)}
+           "ok": all(s["status"] == 0 for s in st),
+           "launch": {"concurrent_buckets": tuned["concurrent"], "wg_per_cu": tuned["wg_per_cu"],
+                      "pilot_default_s": tuned["trials"][0]["seconds"], "pilot_tuned_s": tuned["seconds"]}}
     del plan
     return out, uf
 
@@ -255,7 +265,9 @@ def run_modes(method, descs, nl, u0_host, tf, eps, modes, steps, warmup, floor=F
 
 
 def summarize(res):
-    return {k: {kk: sig(vv) for kk, vv in v.items()} for k, v in res.items()}
+    def one(v):
+        return {kk: one(vv) if isinstance(vv, dict) else sig(vv) for kk, vv in v.items()}
+    return {k: one(v) for k, v in res.items()}
 
 
 def ks_workload(args):

@@ -1,0 +1,74 @@
+"""Launch-shape autotuner for the persistent stepping kernels.
+
+A plan groups its trajectories in size buckets, one persistent-kernel launch
+per bucket.  Two launch choices change throughput and not results:
+
+* whether the buckets run one after another on the caller's stream or
+  overlap on private streams.  In a mixed-size ensemble a bucket's last
+  long trajectories otherwise leave most CUs idle before the next bucket
+  starts;
+* how many workgroups of each bucket stay resident per CU (fewer resident
+  trajectories share a CU's caches and LDS bandwidth with fewer others).
+
+``tune_launch`` times short pilot evolves of the plan's own trajectories and
+keeps the fastest shape.  It tries concurrent (the plan default) and
+sequential buckets first,
+then, bucket by bucket (largest n first), half and a quarter of the resident
+workgroups.  A change is kept only when it is faster by more than
+``min_gain``.  The shape is applied to the plan and returned with the timings
+of every trial.
+"""
+import time
+
+import numpy as np
+import torch
+
+
+def _timed(plan, u0, uf, t0s, tfs, hs, reps):
+    plan.evolve(u0, uf, t0s, tfs, hs)  # warm
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for _ in range(reps):
+        plan.evolve(u0, uf, t0s, tfs, hs)
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t) / reps
+
+
+def tune_launch(plan, u0, t0, tf, h, reps=1, min_gain=0.02):
+    """Tune ``plan``'s launch shape on pilot evolves over [t0, tf] from ``u0``.
+
+    u0: list of device state tensors (the plan's dtype), one per trajectory;
+    t0/tf/h: scalars or per-trajectory sequences (h: first step).
+    Returns {"concurrent", "wg_per_cu", "seconds", "trials"}.
+    """
+    n = plan.ninst
+    t0s = np.broadcast_to(np.asarray(t0, dtype=np.float64), (n,)).tolist()
+    tfs = np.broadcast_to(np.asarray(tf, dtype=np.float64), (n,)).tolist()
+    hs = np.broadcast_to(np.asarray(h, dtype=np.float64), (n,)).tolist()
+    uf = [torch.empty_like(u) for u in u0]
+    buckets = plan.buckets()
+    trials = []
+
+    def trial(conc, wg):
+        plan.set_launch(conc, wg)
+        sec = _timed(plan, u0, uf, t0s, tfs, hs, reps)
+        trials.append({"concurrent": conc, "wg_per_cu": list(wg), "seconds": sec})
+        return sec
+
+    best_wg = [0] * len(buckets)
+    best_conc = True  # the plan's default shape is the first trial
+    best = trial(True, best_wg)
+    if len(buckets) > 1:
+        sec = trial(False, best_wg)
+        if sec < best * (1.0 - min_gain):
+            best, best_conc = sec, False
+    for b, (_l2, _cnt, mx) in enumerate(buckets):
+        for v in sorted({mx // 2, mx // 4} - {0, mx}, reverse=True):
+            wg = list(best_wg)
+            wg[b] = v
+            sec = trial(best_conc, wg)
+            if sec < best * (1.0 - min_gain):
+                best, best_wg = sec, wg
+    plan.set_launch(best_conc, best_wg)
+    return {"concurrent": best_conc, "wg_per_cu": best_wg, "seconds": best, "trials": trials,
+            "buckets": [{"log2n": l2, "count": c, "max_wg_per_cu": m} for l2, c, m in buckets]}

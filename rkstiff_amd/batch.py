@@ -90,6 +90,15 @@ class BatchSolver:
             self._stats = self.plan.stats()
         return uf
 
+    def autotune(self, u0, t0, tf_probe, h=None, reps=1):
+        """Pick the launch shape (rkstiff_amd.autotune) on pilot evolves t0 -> tf_probe."""
+        from .autotune import tune_launch
+        u0 = [u.to(self.dtype).contiguous() for u in u0]
+        if h is None:
+            h = (np.asarray(tf_probe, dtype=np.float64) - np.asarray(t0, dtype=np.float64)) / 100.0
+        self.plan.set_storage(0, 0, 0)
+        return tune_launch(self.plan, u0, t0, tf_probe, h, reps=reps)
+
     # -- learned step-size proposer (rkstiff_amd.learned) ----------------------
     def record(self, cap):
         """Keep up to ``cap`` training records per trajectory during the next evolves."""

@@ -231,6 +231,13 @@ struct rks_plan {
   rks_mlp_dev mlp{};
   float* d_rec = nullptr;
   int rec_cap = 0;
+  // launch shape (rks_plan_set_launch): buckets on private streams, and a
+  // cap on resident workgroups per CU per bucket (0: as many as fit)
+  int concurrent = 1;
+  std::vector<int> wg_per_cu;
+  std::vector<hipStream_t> bstream;
+  std::vector<hipEvent_t> bdone;
+  hipEvent_t fork = nullptr;
 };
 
 static void fill_ctrl(rks_ctrl* c, const rks_config* cfg);
@@ -509,6 +516,9 @@ int rks_plan_destroy(rks_plan* p) {
   (void)hipFree(p->d_queue);
   (void)hipFree(p->d_order);
   (void)hipFree(p->d_rec);
+  for (hipStream_t st : p->bstream) (void)hipStreamDestroy(st);
+  for (hipEvent_t ev : p->bdone) (void)hipEventDestroy(ev);
+  if (p->fork) (void)hipEventDestroy(p->fork);
   for (int l = 0; l < 14; ++l) (void)hipFree(p->d_tw[l]);
   rks2_destroy(p->c2);
   delete p;
@@ -586,7 +596,25 @@ static int launch_all(rks_plan* p, int mode, double h_mode, hipStream_t stream) 
   L.h_mode = h_mode;
   hipLaunchKernelGGL(rks_zero_int, dim3(1), dim3(64), 0, stream, p->d_queue, 64);
   HIPCHK(hipGetLastError());
-  for (size_t b = 0; b < p->bucket_log2.size(); ++b) {
+  const size_t nb = p->bucket_log2.size();
+  const bool conc = p->concurrent && nb > 1;
+  if (conc) {
+    // fork: every bucket stream waits for the caller's stream (queue counters cleared)
+    if (p->bstream.size() < nb) {
+      HIPCHK(hipDeviceSynchronize());
+      if (!p->fork) HIPCHK(hipEventCreateWithFlags(&p->fork, hipEventDisableTiming));
+      while (p->bstream.size() < nb) {
+        hipStream_t st;
+        hipEvent_t ev;
+        HIPCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+        p->bstream.push_back(st);
+        HIPCHK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+        p->bdone.push_back(ev);
+      }
+    }
+    HIPCHK(hipEventRecord(p->fork, stream));
+  }
+  for (size_t b = 0; b < nb; ++b) {
     KernelEntry e;
     const bool learn = p->mlp.on || p->d_rec != nullptr;
     if (!lookup(p->prec, p->nlk, p->bucket_log2[b], learn, &e))
@@ -598,10 +626,20 @@ static int launch_all(rks_plan* p, int mode, double h_mode, hipStream_t stream) 
     int per_cu = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, e.fn, e.threads, e.lds) != hipSuccess || per_cu < 1)
       per_cu = 1;
+    if (b < p->wg_per_cu.size() && p->wg_per_cu[b] > 0 && p->wg_per_cu[b] < per_cu) per_cu = p->wg_per_cu[b];
     int grid = per_cu * p->cu;
     if (grid > L.count) grid = L.count;
     void* args[] = {&L};
-    HIPCHK(hipLaunchKernel(e.fn, dim3(grid), dim3(e.threads), args, (size_t)e.lds, stream));
+    hipStream_t st = stream;
+    if (conc) {
+      st = p->bstream[b];
+      HIPCHK(hipStreamWaitEvent(st, p->fork, 0));
+    }
+    HIPCHK(hipLaunchKernel(e.fn, dim3(grid), dim3(e.threads), args, (size_t)e.lds, st));
+    if (conc) {
+      HIPCHK(hipEventRecord(p->bdone[b], st));
+      HIPCHK(hipStreamWaitEvent(stream, p->bdone[b], 0));
+    }
   }
   return RKS_OK;
 }
@@ -854,6 +892,39 @@ int rks_read_records(rks_plan* p, float* dst_dev, int* counts, hipStream_t strea
                           hipMemcpyDeviceToDevice, stream));
     HIPCHK(hipStreamSynchronize(stream));
   }
+  return RKS_OK;
+}
+
+int rks_plan_buckets(rks_plan* p, int* nbucket, int* log2n, int* count, int* max_wg_per_cu, int cap) {
+  if (!p || !nbucket) return fail(RKS_ERR_INVALID, "null argument");
+  *nbucket = (int)p->bucket_log2.size();
+  for (int b = 0; b < *nbucket && b < cap; ++b) {
+    if (log2n) log2n[b] = p->bucket_log2[b];
+    if (count) count[b] = p->bucket_count[b];
+    if (max_wg_per_cu) {
+      KernelEntry e;
+      int per_cu = 0;
+      if (!lookup(p->prec, p->nlk, p->bucket_log2[b], p->mlp.on || p->d_rec != nullptr, &e) ||
+          hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, e.fn, e.threads, e.lds) != hipSuccess || per_cu < 1)
+        per_cu = 1;
+      max_wg_per_cu[b] = per_cu;
+    }
+  }
+  return RKS_OK;
+}
+
+int rks_plan_set_launch(rks_plan* p, int concurrent, const int* wg_per_cu, int nbucket) {
+  if (!p) return fail(RKS_ERR_INVALID, "null plan");
+  if (p->c2) return fail(RKS_ERR_INVALID, "the 2-D engine has a fixed launch sequence");
+  if (wg_per_cu && nbucket != (int)p->bucket_log2.size())
+    return fail(RKS_ERR_INVALID, "wg_per_cu needs one value per size bucket");
+  p->concurrent = concurrent ? 1 : 0;
+  p->wg_per_cu.assign(p->bucket_log2.size(), 0);
+  if (wg_per_cu)
+    for (int b = 0; b < nbucket; ++b) {
+      if (wg_per_cu[b] < 0) return fail(RKS_ERR_INVALID, "wg_per_cu must be >= 0");
+      p->wg_per_cu[b] = wg_per_cu[b];
+    }
   return RKS_OK;
 }
 

@@ -139,6 +139,24 @@ class Plan:
                                                   float(keep[1])))
         self.proposer = net  # the plan's kernels read the network's device buffers
 
+    # -- launch shape (rkstiff_amd.autotune) ------------------------------------
+    def buckets(self):
+        """[(log2 n, instances, most workgroups per CU)] per size bucket, launch order."""
+        nb = ctypes.c_int()
+        cap = 16
+        l2, cnt, mx = (ctypes.c_int * cap)(), (ctypes.c_int * cap)(), (ctypes.c_int * cap)()
+        _lib.check(self.lib.rks_plan_buckets(self.h, ctypes.byref(nb), l2, cnt, mx, cap))
+        return [(l2[b], cnt[b], mx[b]) for b in range(nb.value)]
+
+    def set_launch(self, concurrent=False, wg_per_cu=None):
+        arr = None
+        n = 0
+        if wg_per_cu is not None:
+            n = len(wg_per_cu)
+            arr = (ctypes.c_int * n)(*[int(v) for v in wg_per_cu])
+        _lib.check(self.lib.rks_plan_set_launch(self.h, 1 if concurrent else 0, arr, n))
+        self.launch = {"concurrent": bool(concurrent), "wg_per_cu": None if wg_per_cu is None else list(wg_per_cu)}
+
     # -- launches --------------------------------------------------------------
     def _check_state(self, tensors):
         for i, t in enumerate(tensors):

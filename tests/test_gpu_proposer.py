@@ -299,3 +299,37 @@ def test_learned_proposer_gain():
     assert r_l <= 0.5 * r_std
     assert a_l <= 0.98 * a_std
     assert e_l.max() <= 10.0 * max(e_fl.max(), e_std.max())
+
+
+def test_launch_shapes_do_not_change_results():
+    """Concurrent buckets and capped residency (the autotuner's choices) give bit-identical
+    trajectories; the tuner returns a shape it measured."""
+    from rkstiff_amd.autotune import tune_launch
+    ens = ks_ensemble(96, seed=41, sizes=(256, 1024, 4096))
+    cfg = SolverConfig(epsilon=1e-5)
+    descs = [nl.descriptor(torch.as_tensor(L)) for (_, _, L, nl, _) in ens]
+    plan = Plan("IF34", ens[0][3], _lib.F64, cfg, None, descs)
+    u0 = [torch.as_tensor(e[4]).cuda() for e in ens]
+    n = len(ens)
+
+    def run():
+        uf = [torch.empty_like(u) for u in u0]
+        plan.evolve(u0, uf, [0.0] * n, [5.0] * n, [0.05] * n)
+        st = plan.stats()
+        return torch.stack([torch.view_as_real(u).reshape(-1)[:8] for u in uf]).cpu(), \
+            [u.cpu() for u in uf], [(s["attempts"], s["steps"]) for s in st]
+
+    _, ref, cnt = run()
+    buckets = plan.buckets()
+    assert [b[0] for b in buckets] == [12, 10, 8] and sum(b[1] for b in buckets) == n
+    for conc, wg in ((True, None), (False, [1] * len(buckets)), (True, [max(1, b[2] // 2) for b in buckets])):
+        plan.set_launch(conc, wg)
+        _, out, c2 = run()
+        assert c2 == cnt
+        for a, b in zip(out, ref):
+            assert torch.equal(a, b)
+    info = tune_launch(plan, u0, 0.0, 1.0, 0.01)
+    print("tuned", info["concurrent"], info["wg_per_cu"], [round(t["seconds"], 4) for t in info["trials"]])
+    assert info["seconds"] == min(t["seconds"] for t in info["trials"]) or info["seconds"] <= info["trials"][0]["seconds"]
+    with pytest.raises(ValueError):
+        plan.set_launch(False, [1])
