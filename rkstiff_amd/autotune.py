@@ -48,6 +48,11 @@ def tune_launch(plan, u0, t0, tf, h, reps=1, min_gain=0.02):
     uf = [torch.empty_like(u) for u in u0]
     buckets = plan.buckets()
     trials = []
+    if not buckets:
+        # the 2-D engine: a fixed kernel sequence, nothing to choose
+        sec = _timed(plan, u0, uf, t0s, tfs, hs, reps)
+        return {"concurrent": None, "wg_per_cu": [], "seconds": sec,
+                "trials": [{"concurrent": None, "wg_per_cu": [], "seconds": sec}], "buckets": []}
 
     def trial(conc, wg):
         plan.set_launch(conc, wg)
