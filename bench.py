@@ -34,8 +34,8 @@ Also measured and reported under "workloads" (one timed pass each):
                 regenerations, error vs FP64 with the step-sequence floor)
 
 Every measured plan first picks its launch shape (rkstiff_amd.autotune: size
-buckets sequential or overlapped on private streams, resident workgroups per
-CU) from pilot evolves to tf / 20; the line reports the shape and the pilot
+buckets sequential or overlapped on private streams, FFT tile, resident
+workgroups per CU) from pilot evolves to tf / 20; the line reports the shape and the pilot
 times.  Tuning is plan set-up and stays outside the timed region.
 
 Reported beside the value:
@@ -234,6 +234,7 @@ def measure(method, descs, nl, u0_host, tf, eps, mode, steps, warmup):
            "achieved_GBs": alg / (k_ms * 1e-3) / 1e9, "attempts": float(np.mean([s["attempts"] for s in st])),
            "ok": all(s["status"] == 0 for s in st),
            "launch": {"concurrent_buckets": tuned["concurrent"], "wg_per_cu": tuned["wg_per_cu"],
+                      "fft_tile": tuned["tile"],
                       "pilot_default_s": tuned["trials"][0]["seconds"], "pilot_tuned_s": tuned["seconds"]}}
     del plan
     return out, uf

@@ -144,14 +144,18 @@ int rks_error_s(rks_plan* p, int inst, const void* u_new, const void* err, doubl
  * has no launches; set by the autotuner, rkstiff_amd.autotune).  Instances are
  * grouped in size buckets (one kernel launch each, largest n first).
  * rks_plan_buckets: nbucket and, per bucket (up to cap), log2 n, the instance
- * count and the most resident workgroups per CU the kernel allows.
+ * count, the most resident workgroups per CU the kernel allows and the number
+ * of FFT tiles available (2 for n = 512 .. 4096: tile 1 runs twice the threads
+ * per trajectory, half the modes per thread).
  * rks_plan_set_launch: concurrent != 0 (the default) launches the buckets on
  * private streams forked from and joined back to the caller's stream, so a
  * bucket's tail overlaps the others; 0 launches them in order on the caller's
  * stream.  wg_per_cu[b] (NULL: all 0) caps bucket b's resident workgroups per
- * CU, 0 = as many as fit.  Results do not depend on the shape. */
-int rks_plan_buckets(rks_plan* p, int* nbucket, int* log2n, int* count, int* max_wg_per_cu, int cap);
-int rks_plan_set_launch(rks_plan* p, int concurrent, const int* wg_per_cu, int nbucket);
+ * CU, 0 = as many as fit; tile[b] (NULL: all 0) picks bucket b's FFT tile
+ * (the proposer variant always runs tile 0).  Results do not depend on the
+ * shape or the tile. */
+int rks_plan_buckets(rks_plan* p, int* nbucket, int* log2n, int* count, int* max_wg_per_cu, int* ntile, int cap);
+int rks_plan_set_launch(rks_plan* p, int concurrent, const int* wg_per_cu, const int* tile, int nbucket);
 
 /* Synchronise `stream` and read per-instance counters/status (host array of ninst). */
 int rks_get_stats(rks_plan* p, rks_stats* out, hipStream_t stream);

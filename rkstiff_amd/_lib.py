@@ -147,8 +147,8 @@ def load(path=None):
         lib.rks_hankel_destroy.argtypes = [vp]
         lib.rks_hankel_apply.argtypes = [vp, ip, vp, vp, ip, ip, vp]
         lib.rks_hankel_cubic.argtypes = [vp, dp, vp, vp, ip, vp]
-        lib.rks_plan_buckets.argtypes = [vp, P(ip), P(ip), P(ip), P(ip), ip]
-        lib.rks_plan_set_launch.argtypes = [vp, ip, P(ip), ip]
+        lib.rks_plan_buckets.argtypes = [vp, P(ip), P(ip), P(ip), P(ip), P(ip), ip]
+        lib.rks_plan_set_launch.argtypes = [vp, ip, P(ip), P(ip), ip]
         lib.rks_last_error.restype = ctypes.c_char_p
         for name in EXPORTS:
             if name not in ("rks_last_error",):

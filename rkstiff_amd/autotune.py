@@ -8,13 +8,16 @@ per bucket.  Two launch choices change throughput and not results:
   long trajectories otherwise leave most CUs idle before the next bucket
   starts;
 * how many workgroups of each bucket stay resident per CU (fewer resident
-  trajectories share a CU's caches and LDS bandwidth with fewer others).
+  trajectories share a CU's caches and LDS bandwidth with fewer others);
+* the FFT tile of each bucket where a second one exists (n = 512 .. 4096):
+  twice the threads per trajectory, half the modes per thread in the stage
+  passes and transforms.
 
 ``tune_launch`` times short pilot evolves of the plan's own trajectories and
 keeps the fastest shape.  It tries concurrent (the plan default) and
 sequential buckets first,
-then, bucket by bucket (largest n first), half and a quarter of the resident
-workgroups.  A change is kept only when it is faster by more than
+then, bucket by bucket (largest n first), the second tile and then half and
+a quarter of the resident workgroups.  A change is kept only when it is faster by more than
 ``min_gain``.  The shape is applied to the plan and returned with the timings
 of every trial.
 """
@@ -51,29 +54,42 @@ def tune_launch(plan, u0, t0, tf, h, reps=1, min_gain=0.02):
     if not buckets:
         # the 2-D engine: a fixed kernel sequence, nothing to choose
         sec = _timed(plan, u0, uf, t0s, tfs, hs, reps)
-        return {"concurrent": None, "wg_per_cu": [], "seconds": sec,
-                "trials": [{"concurrent": None, "wg_per_cu": [], "seconds": sec}], "buckets": []}
+        return {"concurrent": None, "wg_per_cu": [], "tile": [], "seconds": sec,
+                "trials": [{"concurrent": None, "wg_per_cu": [], "tile": [], "seconds": sec}], "buckets": []}
 
-    def trial(conc, wg):
-        plan.set_launch(conc, wg)
+    def trial(conc, wg, ti):
+        plan.set_launch(conc, wg, ti)
         sec = _timed(plan, u0, uf, t0s, tfs, hs, reps)
-        trials.append({"concurrent": conc, "wg_per_cu": list(wg), "seconds": sec})
+        trials.append({"concurrent": conc, "wg_per_cu": list(wg), "tile": list(ti), "seconds": sec})
         return sec
 
-    best_wg = [0] * len(buckets)
+    nb = len(buckets)
+    best_wg, best_ti = [0] * nb, [0] * nb
     best_conc = True  # the plan's default shape is the first trial
-    best = trial(True, best_wg)
-    if len(buckets) > 1:
-        sec = trial(False, best_wg)
+    best = trial(True, best_wg, best_ti)
+    if nb > 1:
+        sec = trial(False, best_wg, best_ti)
         if sec < best * (1.0 - min_gain):
             best, best_conc = sec, False
-    for b, (_l2, _cnt, mx) in enumerate(buckets):
+    for b, (_l2, _cnt, _mx, ntile) in enumerate(buckets):
+        if ntile > 1:
+            ti = list(best_ti)
+            ti[b] = 1
+            sec = trial(best_conc, best_wg, ti)
+            if sec < best * (1.0 - min_gain):
+                best, best_ti = sec, ti
+    plan.set_launch(best_conc, best_wg, best_ti)
+    for b in range(nb):
+        # resident-workgroup limit of the chosen tile
+        mx = plan.buckets()[b][2] if best_ti[b] == 0 else None
+        if mx is None:
+            continue
         for v in sorted({mx // 2, mx // 4} - {0, mx}, reverse=True):
             wg = list(best_wg)
             wg[b] = v
-            sec = trial(best_conc, wg)
+            sec = trial(best_conc, wg, best_ti)
             if sec < best * (1.0 - min_gain):
                 best, best_wg = sec, wg
-    plan.set_launch(best_conc, best_wg)
-    return {"concurrent": best_conc, "wg_per_cu": best_wg, "seconds": best, "trials": trials,
-            "buckets": [{"log2n": l2, "count": c, "max_wg_per_cu": m} for l2, c, m in buckets]}
+    plan.set_launch(best_conc, best_wg, best_ti)
+    return {"concurrent": best_conc, "wg_per_cu": best_wg, "tile": best_ti, "seconds": best, "trials": trials,
+            "buckets": [{"log2n": l2, "count": c, "max_wg_per_cu": m, "tiles": t} for l2, c, m, t in buckets]}

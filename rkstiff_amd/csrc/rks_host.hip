@@ -139,17 +139,28 @@ struct KernelEntry {
   int lds;
 };
 
-template <typename R, typename RN, int LOG2N, int NLK, bool LE>
+template <typename R, typename RN, int LOG2N, int NLK, bool LE, int TI = 0>
 KernelEntry entry_of() {
   KernelEntry e;
-  e.fn = (const void*)&rks_kernel<R, RN, LOG2N, NLK, LE>;
-  e.threads = rks_threads(LOG2N, NLK);
+  e.fn = (const void*)&rks_kernel<R, RN, LOG2N, NLK, LE, TI>;
+  e.threads = rks_threads(LOG2N, NLK) << TI;
   e.lds = rks_lds_bytes<R, LOG2N>();
   return e;
 }
 
 template <typename R, typename RN, int NLK>
-bool lookup_nlk(int log2n, bool learn, KernelEntry* e) {
+bool lookup_nlk(int log2n, bool learn, int tile, KernelEntry* e) {
+  if (tile == 1) {
+    if (learn) return false;
+    switch (log2n) {
+      case 9: *e = entry_of<R, RN, 9, NLK, false, 1>(); return true;
+      case 10: *e = entry_of<R, RN, 10, NLK, false, 1>(); return true;
+      case 11: *e = entry_of<R, RN, 11, NLK, false, 1>(); return true;
+      case 12: *e = entry_of<R, RN, 12, NLK, false, 1>(); return true;
+      default: return false;
+    }
+  }
+  if (tile != 0) return false;
   if (learn) {
     switch (log2n) {
       case 7: *e = entry_of<R, RN, 7, NLK, true>(); return true;
@@ -177,14 +188,15 @@ bool lookup_nlk(int log2n, bool learn, KernelEntry* e) {
   }
 }
 
-// learn: the proposer variant (a proposer or training records are attached)
-bool lookup(int prec, int nlk, int log2n, bool learn, KernelEntry* e) {
-  if (prec == RKS_F64) return nlk == NLK_ADVECT ? lookup_nlk<double, double, NLK_ADVECT>(log2n, learn, e)
-                                                : lookup_nlk<double, double, NLK_CUBIC>(log2n, learn, e);
-  if (prec == RKS_MIXED) return nlk == NLK_ADVECT ? lookup_nlk<double, float, NLK_ADVECT>(log2n, learn, e)
-                                                  : lookup_nlk<double, float, NLK_CUBIC>(log2n, learn, e);
-  return nlk == NLK_ADVECT ? lookup_nlk<float, float, NLK_ADVECT>(log2n, learn, e)
-                           : lookup_nlk<float, float, NLK_CUBIC>(log2n, learn, e);
+// learn: the proposer variant (a proposer or training records are attached);
+// tile: 0 the default tile, 1 twice the threads per trajectory (n = 2^9 .. 2^12)
+bool lookup(int prec, int nlk, int log2n, bool learn, KernelEntry* e, int tile = 0) {
+  if (prec == RKS_F64) return nlk == NLK_ADVECT ? lookup_nlk<double, double, NLK_ADVECT>(log2n, learn, tile, e)
+                                                : lookup_nlk<double, double, NLK_CUBIC>(log2n, learn, tile, e);
+  if (prec == RKS_MIXED) return nlk == NLK_ADVECT ? lookup_nlk<double, float, NLK_ADVECT>(log2n, learn, tile, e)
+                                                  : lookup_nlk<double, float, NLK_CUBIC>(log2n, learn, tile, e);
+  return nlk == NLK_ADVECT ? lookup_nlk<float, float, NLK_ADVECT>(log2n, learn, tile, e)
+                           : lookup_nlk<float, float, NLK_CUBIC>(log2n, learn, tile, e);
 }
 
 int ilog2(int n) {
@@ -235,6 +247,7 @@ struct rks_plan {
   // cap on resident workgroups per CU per bucket (0: as many as fit)
   int concurrent = 1;
   std::vector<int> wg_per_cu;
+  std::vector<int> tile;  // per bucket: 0 default tile, 1 twice the threads
   std::vector<hipStream_t> bstream;
   std::vector<hipEvent_t> bdone;
   hipEvent_t fork = nullptr;
@@ -460,6 +473,8 @@ int rks_plan_create(int method, int nl_kind, int precision, const rks_config* cf
       KernelEntry el;
       if (lookup(precision, nl_kind, l, true, &el))
         HIPCHK(hipFuncSetAttribute(el.fn, hipFuncAttributeMaxDynamicSharedMemorySize, el.lds));
+      if (lookup(precision, nl_kind, l, false, &el, 1))
+        HIPCHK(hipFuncSetAttribute(el.fn, hipFuncAttributeMaxDynamicSharedMemorySize, el.lds));
       const int N = 1 << l;
       HIPCHK(hipMalloc(&p->d_tw[l], (size_t)N * p->esize));
       if (precision != RKS_F32)
@@ -617,8 +632,9 @@ static int launch_all(rks_plan* p, int mode, double h_mode, hipStream_t stream) 
   for (size_t b = 0; b < nb; ++b) {
     KernelEntry e;
     const bool learn = p->mlp.on || p->d_rec != nullptr;
-    if (!lookup(p->prec, p->nlk, p->bucket_log2[b], learn, &e))
-      return fail(RKS_ERR_INVALID, "no proposer kernel for this grid size");
+    const int ti = (!learn && b < p->tile.size()) ? p->tile[b] : 0;
+    if (!lookup(p->prec, p->nlk, p->bucket_log2[b], learn, &e, ti))
+      return fail(RKS_ERR_INVALID, "no kernel for this grid size, variant and tile");
     L.order = p->d_order + p->bucket_start[b];
     L.count = p->bucket_count[b];
     L.queue = p->d_queue + b;
@@ -895,12 +911,16 @@ int rks_read_records(rks_plan* p, float* dst_dev, int* counts, hipStream_t strea
   return RKS_OK;
 }
 
-int rks_plan_buckets(rks_plan* p, int* nbucket, int* log2n, int* count, int* max_wg_per_cu, int cap) {
+int rks_plan_buckets(rks_plan* p, int* nbucket, int* log2n, int* count, int* max_wg_per_cu, int* ntile, int cap) {
   if (!p || !nbucket) return fail(RKS_ERR_INVALID, "null argument");
   *nbucket = (int)p->bucket_log2.size();
   for (int b = 0; b < *nbucket && b < cap; ++b) {
     if (log2n) log2n[b] = p->bucket_log2[b];
     if (count) count[b] = p->bucket_count[b];
+    if (ntile) {
+      KernelEntry e1;
+      ntile[b] = lookup(p->prec, p->nlk, p->bucket_log2[b], false, &e1, 1) ? 2 : 1;
+    }
     if (max_wg_per_cu) {
       KernelEntry e;
       int per_cu = 0;
@@ -913,18 +933,27 @@ int rks_plan_buckets(rks_plan* p, int* nbucket, int* log2n, int* count, int* max
   return RKS_OK;
 }
 
-int rks_plan_set_launch(rks_plan* p, int concurrent, const int* wg_per_cu, int nbucket) {
+int rks_plan_set_launch(rks_plan* p, int concurrent, const int* wg_per_cu, const int* tile, int nbucket) {
   if (!p) return fail(RKS_ERR_INVALID, "null plan");
   if (p->c2) return fail(RKS_ERR_INVALID, "the 2-D engine has a fixed launch sequence");
-  if (wg_per_cu && nbucket != (int)p->bucket_log2.size())
-    return fail(RKS_ERR_INVALID, "wg_per_cu needs one value per size bucket");
-  p->concurrent = concurrent ? 1 : 0;
-  p->wg_per_cu.assign(p->bucket_log2.size(), 0);
-  if (wg_per_cu)
-    for (int b = 0; b < nbucket; ++b) {
+  const int nb = (int)p->bucket_log2.size();
+  if ((wg_per_cu || tile) && nbucket != nb) return fail(RKS_ERR_INVALID, "one value per size bucket needed");
+  std::vector<int> w(nb, 0), ti(nb, 0);
+  for (int b = 0; b < nb; ++b) {
+    if (wg_per_cu) {
       if (wg_per_cu[b] < 0) return fail(RKS_ERR_INVALID, "wg_per_cu must be >= 0");
-      p->wg_per_cu[b] = wg_per_cu[b];
+      w[b] = wg_per_cu[b];
     }
+    if (tile) {
+      KernelEntry e1;
+      if (tile[b] != 0 && !(tile[b] == 1 && lookup(p->prec, p->nlk, p->bucket_log2[b], false, &e1, 1)))
+        return fail(RKS_ERR_INVALID, "tile 1 exists for n = 512 .. 4096 only");
+      ti[b] = tile[b];
+    }
+  }
+  p->concurrent = concurrent ? 1 : 0;
+  p->wg_per_cu = w;
+  p->tile = ti;
   return RKS_OK;
 }
 

@@ -1131,9 +1131,12 @@ __host__ __device__ constexpr int rks_min_waves(int log2n) { return log2n >= 13 
 // LEARN: the variant with the learned step-size proposer and its training
 // records; the plain variant carries none of that code, so the stepping loop
 // keeps its register allocation.
-template <typename R, typename RN, int LOG2N, int NLK, bool LEARN>
-__global__ __launch_bounds__(rks_threads(LOG2N, NLK), rks_min_waves(LOG2N)) void rks_kernel(rks_launch P) {
-  constexpr int T = rks_threads(LOG2N, NLK);
+// TILE 1: twice the threads per trajectory (half the modes per thread in the
+// stage passes and FFTs), where every transform keeps >= 2 elements per thread.
+template <typename R, typename RN, int LOG2N, int NLK, bool LEARN, int TILE>
+__global__ __launch_bounds__(rks_threads(LOG2N, NLK) << TILE, rks_min_waves(LOG2N)) void rks_kernel(rks_launch P) {
+  constexpr int T = rks_threads(LOG2N, NLK) << TILE;
+  static_assert(T <= 512 && T <= (NLK == NLK_ADVECT ? (1 << LOG2N) / 4 : (1 << LOG2N) / 2), "tile too wide");
   extern __shared__ __align__(16) unsigned char rks_smem[];
   __shared__ double red[2 * (T / 64 + 1)];
   __shared__ int s_item;
@@ -1163,11 +1166,14 @@ __global__ __launch_bounds__(rks_threads(LOG2N, NLK), rks_min_waves(LOG2N)) void
 // Explicit instantiations of one precision x nonlinearity (rks_k_*.hip): every
 // size, and the proposer variant for the sizes whose workgroups can be whole
 // waves (n >= 128; the host refuses smaller rfft grids).
-#define RKS_KERNEL_SIZES(X, R, RN, NLK)                                                              \
-  X(R, RN, 4, NLK, false) X(R, RN, 5, NLK, false) X(R, RN, 6, NLK, false) X(R, RN, 7, NLK, false)   \
-  X(R, RN, 8, NLK, false) X(R, RN, 9, NLK, false) X(R, RN, 10, NLK, false) X(R, RN, 11, NLK, false) \
-  X(R, RN, 12, NLK, false) X(R, RN, 13, NLK, false) X(R, RN, 7, NLK, true) X(R, RN, 8, NLK, true)   \
-  X(R, RN, 9, NLK, true) X(R, RN, 10, NLK, true) X(R, RN, 11, NLK, true) X(R, RN, 12, NLK, true)    \
-  X(R, RN, 13, NLK, true)
-#define RKS_INSTANTIATE(R, RN, L, K, LE) template __global__ void rks_kernel<R, RN, L, K, LE>(rks_launch);
-#define RKS_DECLARE(R, RN, L, K, LE) extern template __global__ void rks_kernel<R, RN, L, K, LE>(rks_launch);
+// The second tile (TILE 1) exists for n = 2^9 .. 2^12.
+#define RKS_KERNEL_SIZES(X, R, RN, NLK)                                                                      \
+  X(R, RN, 4, NLK, false, 0) X(R, RN, 5, NLK, false, 0) X(R, RN, 6, NLK, false, 0)                          \
+  X(R, RN, 7, NLK, false, 0) X(R, RN, 8, NLK, false, 0) X(R, RN, 9, NLK, false, 0)                          \
+  X(R, RN, 10, NLK, false, 0) X(R, RN, 11, NLK, false, 0) X(R, RN, 12, NLK, false, 0)                       \
+  X(R, RN, 13, NLK, false, 0) X(R, RN, 7, NLK, true, 0) X(R, RN, 8, NLK, true, 0) X(R, RN, 9, NLK, true, 0) \
+  X(R, RN, 10, NLK, true, 0) X(R, RN, 11, NLK, true, 0) X(R, RN, 12, NLK, true, 0)                          \
+  X(R, RN, 13, NLK, true, 0) X(R, RN, 9, NLK, false, 1) X(R, RN, 10, NLK, false, 1)                         \
+  X(R, RN, 11, NLK, false, 1) X(R, RN, 12, NLK, false, 1)
+#define RKS_INSTANTIATE(R, RN, L, K, LE, TI) template __global__ void rks_kernel<R, RN, L, K, LE, TI>(rks_launch);
+#define RKS_DECLARE(R, RN, L, K, LE, TI) extern template __global__ void rks_kernel<R, RN, L, K, LE, TI>(rks_launch);

@@ -141,21 +141,22 @@ class Plan:
 
     # -- launch shape (rkstiff_amd.autotune) ------------------------------------
     def buckets(self):
-        """[(log2 n, instances, most workgroups per CU)] per size bucket, launch order."""
+        """[(log2 n, instances, most workgroups per CU, FFT tiles available)] per size bucket,
+        launch order."""
         nb = ctypes.c_int()
         cap = 16
-        l2, cnt, mx = (ctypes.c_int * cap)(), (ctypes.c_int * cap)(), (ctypes.c_int * cap)()
-        _lib.check(self.lib.rks_plan_buckets(self.h, ctypes.byref(nb), l2, cnt, mx, cap))
-        return [(l2[b], cnt[b], mx[b]) for b in range(nb.value)]
+        l2, cnt, mx, nt = [(ctypes.c_int * cap)() for _ in range(4)]
+        _lib.check(self.lib.rks_plan_buckets(self.h, ctypes.byref(nb), l2, cnt, mx, nt, cap))
+        return [(l2[b], cnt[b], mx[b], nt[b]) for b in range(nb.value)]
 
-    def set_launch(self, concurrent=False, wg_per_cu=None):
-        arr = None
-        n = 0
-        if wg_per_cu is not None:
-            n = len(wg_per_cu)
-            arr = (ctypes.c_int * n)(*[int(v) for v in wg_per_cu])
-        _lib.check(self.lib.rks_plan_set_launch(self.h, 1 if concurrent else 0, arr, n))
-        self.launch = {"concurrent": bool(concurrent), "wg_per_cu": None if wg_per_cu is None else list(wg_per_cu)}
+    def set_launch(self, concurrent=True, wg_per_cu=None, tile=None):
+        def arr(v):
+            return None if v is None else (ctypes.c_int * len(v))(*[int(x) for x in v])
+        n = len(wg_per_cu) if wg_per_cu is not None else (len(tile) if tile is not None else 0)
+        _lib.check(self.lib.rks_plan_set_launch(self.h, 1 if concurrent else 0, arr(wg_per_cu), arr(tile), n))
+        self.launch = {"concurrent": bool(concurrent),
+                       "wg_per_cu": None if wg_per_cu is None else list(wg_per_cu),
+                       "tile": None if tile is None else list(tile)}
 
     # -- launches --------------------------------------------------------------
     def _check_state(self, tensors):
