@@ -291,6 +291,7 @@ __global__ __launch_bounds__(256) void k2_coef(rks2_launch P, int M) {
     coef_mode<R, RN>(*P.tab, *P.ctl, in->L, in->L_complex, m, M, in->h, cbase);
 }
 
+
 // ---------------------------------------------------------------------------
 // norm: masked sums |u_new|^2, |err|^2 over modes with |u_new|/max > cutoff
 // ---------------------------------------------------------------------------

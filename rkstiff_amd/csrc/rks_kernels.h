@@ -216,25 +216,30 @@ __device__ __forceinline__ void frac_coefs(TB& tb, int f, const z64 (&v)[4], dou
   }
 }
 
-// All coefficients of one mode by one thread (generic-path kernel).
+// All coefficients of one mode by one thread (generic-path and 2-D
+// kernels).  The tableau and controller references must be uniform (one
+// object for the launch); they are read through the scalar
+// constant-address-space view.
 template <typename RS, typename RN>
-__device__ void coef_mode(const rks_tableau& tb, const rks_ctrl& ct, const double* __restrict__ L,
+__device__ void coef_mode(const rks_tableau& tb_in, const rks_ctrl& ct_in, const double* __restrict__ L,
                           int L_complex, int m, int M, double h, char* __restrict__ cbase) {
 #pragma clang fp contract(off)
+  rks_ctab& tb = *rks_uniform_const<rks_ctab>(&tb_in);
+  rks_cctl& ct = *rks_uniform_const<rks_cctl>(&ct_in);
   const int cplx = L_complex || tb.uses_contour;
   const z64 z = mode_z(L, L_complex, m, h);
   const bool small = tb.uses_contour && hypot(z.re, z.im) < ct.modecutoff;
   for (int f = 0; f < tb.nfrac; ++f) {
-    const rks_frac fr = tb.frac[f];
+    const int fnum = tb.frac[f].fnum, fden = tb.frac[f].fden, fmask = tb.frac[f].mask;
     z64 v[4];
     v[1] = v[2] = v[3] = mkc(0.0, 0.0);
-    frac_direct(fr.fnum, fr.fden, fr.mask, z, h, cplx, v);
-    if (small && (fr.mask & 14)) {
+    frac_direct(fnum, fden, fmask, z, h, cplx, v);
+    if (small && (fmask & 14)) {
       z64 acc[4];
       acc[1] = acc[2] = acc[3] = mkc(0.0, 0.0);
       for (int j = 0; j < ct.contour_points; ++j) {
         const z64 nd = contour_node(ct.contour_points, ct.contour_radius, j);
-        frac_contour_add(fr.fnum, fr.fden, fr.mask, mkc(z.re + nd.re, z.im + nd.im), acc);
+        frac_contour_add(fnum, fden, fmask, mkc(z.re + nd.re, z.im + nd.im), acc);
       }
       for (int r = 1; r <= 3; ++r) {
         const z64 a = r == 1 ? acc[1] : (r == 2 ? acc[2] : acc[3]);
@@ -248,38 +253,16 @@ __device__ void coef_mode(const rks_tableau& tb, const rks_ctrl& ct, const doubl
   }
 }
 
-// All coefficients of an instance by one workgroup: closed forms per thread,
-// then the small modes' contour means spread over groups of up to 32 lanes
-// (one lane per node, deterministic shuffle reduction).  `list` is LDS
-// scratch for up to M ints; *count is an LDS counter.
-template <typename RS, typename RN, int T, int MPT>
-__device__ __attribute__((noinline)) void coef_update(const rks_tableau& tb_in, const rks_ctrl& ct_in,
-                                                      const double* __restrict__ L, int L_complex, int M, double h,
-                                                      char* __restrict__ cbase, int* list, int* count) {
+// Coefficients of the listed small modes (|z| < modecutoff): groups of up to
+// 32 lanes per mode, one lane per contour node, deterministic shuffle sums.
+// All T threads of the workgroup call this with the same list.
+template <typename RS, typename RN, int T, typename TB, typename CT>
+__device__ __forceinline__ void coef_small_group(TB& tb, CT& ct, const double* __restrict__ L, int L_complex,
+                                                 int M, double h, char* __restrict__ cbase, const int* list,
+                                                 int nsmall) {
 #pragma clang fp contract(off)
-  rks_ctab& tb = *rks_uniform_const<rks_ctab>(&tb_in);
-  rks_cctl& ct = *rks_uniform_const<rks_cctl>(&ct_in);
   const int t = threadIdx.x;
   const int cplx = L_complex || tb.uses_contour;
-  if (t == 0) *count = 0;
-  __syncthreads();
-  for (int i = 0; i < MPT; ++i) {
-    const int m = t + i * T;
-    if (m >= M) break;
-    const z64 z = mode_z(L, L_complex, m, h);
-    if (tb.uses_contour && hypot(z.re, z.im) < ct.modecutoff) {
-      list[atomicAdd(count, 1)] = m;
-      continue;
-    }
-    for (int f = 0; f < tb.nfrac; ++f) {
-      z64 v[4];
-      v[1] = v[2] = v[3] = mkc(0.0, 0.0);
-      frac_direct(tb.frac[f].fnum, tb.frac[f].fden, tb.frac[f].mask, z, h, cplx, v);
-      frac_coefs<RS, RN>(tb, f, v, h, cplx, m, (size_t)M, cbase);
-    }
-  }
-  __syncthreads();
-  const int nsmall = *count;
   if (nsmall > 0) {
     constexpr int GS = T < 32 ? T : 32;
     constexpr int NG = T / GS;
@@ -320,6 +303,40 @@ __device__ __attribute__((noinline)) void coef_update(const rks_tableau& tb_in, 
       }
     }
   }
+}
+
+// All coefficients of an instance by one workgroup: closed forms per thread,
+// then the small modes' contour means spread over groups of up to 32 lanes
+// (one lane per node, deterministic shuffle reduction).  `list` is LDS
+// scratch for up to M ints; *count is an LDS counter.
+template <typename RS, typename RN, int T, int MPT>
+__device__ __attribute__((noinline)) void coef_update(const rks_tableau& tb_in, const rks_ctrl& ct_in,
+                                                      const double* __restrict__ L, int L_complex, int M, double h,
+                                                      char* __restrict__ cbase, int* list, int* count) {
+#pragma clang fp contract(off)
+  rks_ctab& tb = *rks_uniform_const<rks_ctab>(&tb_in);
+  rks_cctl& ct = *rks_uniform_const<rks_cctl>(&ct_in);
+  const int t = threadIdx.x;
+  const int cplx = L_complex || tb.uses_contour;
+  if (t == 0) *count = 0;
+  __syncthreads();
+  for (int i = 0; i < MPT; ++i) {
+    const int m = t + i * T;
+    if (m >= M) break;
+    const z64 z = mode_z(L, L_complex, m, h);
+    if (tb.uses_contour && hypot(z.re, z.im) < ct.modecutoff) {
+      list[atomicAdd(count, 1)] = m;
+      continue;
+    }
+    for (int f = 0; f < tb.nfrac; ++f) {
+      z64 v[4];
+      v[1] = v[2] = v[3] = mkc(0.0, 0.0);
+      frac_direct(tb.frac[f].fnum, tb.frac[f].fden, tb.frac[f].mask, z, h, cplx, v);
+      frac_coefs<RS, RN>(tb, f, v, h, cplx, m, (size_t)M, cbase);
+    }
+  }
+  __syncthreads();
+  coef_small_group<RS, RN, T>(tb, ct, L, L_complex, M, h, cbase, list, *count);
   __syncthreads();
 }
 
