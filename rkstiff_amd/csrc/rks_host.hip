@@ -49,6 +49,13 @@ __global__ void rks_combine_kernel(rks_combine_args A) {
   }
 }
 
+// real coefficient storage -> complex (the copy-out of rks_coeffs)
+template <typename R>
+__global__ void rks_real_to_complex(const R* a, cx<R>* b, size_t n) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    b[i] = mkc(a[i], (R)0);
+}
+
 // s = safety * (eps ||u[idx]|| / ||err[idx]||)^(1/q), idx = |u|/max|u| > cutoff; one workgroup
 template <typename R>
 __global__ __launch_bounds__(1024) void rks_norm_kernel(const cx<R>* u, const cx<R>* err, int M, double eps,
@@ -210,6 +217,7 @@ int ilog2(int n) {
 struct rks_plan {
   int method = 0, nlk = 0, prec = 0, ninst = 0, device = 0, cu = 256;
   int esize = 16;  // state element bytes (complex)
+  int coef_real = 0;  // coefficients stored real (RKS_CST_*_REAL)
   int nsize = 16;  // NL / coefficient storage element bytes (complex)
   size_t coef_bytes_per_mode = 0;
   rks_tableau tab;
@@ -304,19 +312,30 @@ int rks_plan_create(int method, int nl_kind, int precision, const rks_config* cf
   p->esize = precision == RKS_F32 ? 8 : 16;
   p->nsize = precision == RKS_F64 ? 16 : 8;
   // coefficient layout: index order for uniform precision; propagators (state
-  // precision) first, then the rest, for mixed precision
+  // precision) first, then the rest, for mixed precision.  FP32 and mixed
+  // plans whose lin_op is real everywhere store every coefficient real (z = h L
+  // is real, so are exp and the psi functions; the contour means' imaginary
+  // parts are rounding) at half the bytes; FP64 plans keep the reference's
+  // complex coefficients.
   {
+    bool real = precision != RKS_F64 && nl_kind != RKS_NL_NONE;
+    for (int i = 0; i < ninst && real; ++i) real = !desc[i].L_is_complex;
+    p->coef_real = real ? 1 : 0;
+    const int rs_b = (precision == RKS_MIXED ? 16 : p->esize) / (real ? 2 : 1);
+    const int rn_b = (precision == RKS_MIXED ? 8 : p->esize) / (real ? 2 : 1);
     int n_rs = 0;
     for (int c = 0; c < p->tab.ncoef; ++c) n_rs += p->tab.coef_rs[c] ? 1 : 0;
     int rank_rs = 0, rank_rn = 0;
     for (int c = 0; c < p->tab.ncoef; ++c) {
-      if (precision != RKS_MIXED) p->tab.coef_off[c] = c * p->esize;
-      else if (p->tab.coef_rs[c]) p->tab.coef_off[c] = 16 * rank_rs++;
-      else p->tab.coef_off[c] = 16 * n_rs + 8 * rank_rn++;
-      if (precision != RKS_MIXED) p->tab.coef_rs[c] = 1;
+      int rs = p->tab.coef_rs[c] ? 1 : 0;
+      if (precision != RKS_MIXED) rs = 1;  // one storage precision
+      if (precision != RKS_MIXED) p->tab.coef_off[c] = c * rs_b;
+      else if (rs) p->tab.coef_off[c] = rs_b * rank_rs++;
+      else p->tab.coef_off[c] = rs_b * n_rs + rn_b * rank_rn++;
+      p->tab.coef_rs[c] = real ? (rs ? RKS_CST_RS_REAL : RKS_CST_RN_REAL) : (rs ? RKS_CST_RS : RKS_CST_RN);
     }
-    p->coef_bytes_per_mode = precision == RKS_MIXED ? (size_t)(16 * n_rs + 8 * (p->tab.ncoef - n_rs))
-                                                    : (size_t)p->tab.ncoef * p->esize;
+    p->coef_bytes_per_mode = precision == RKS_MIXED ? (size_t)(rs_b * n_rs + rn_b * (p->tab.ncoef - n_rs))
+                                                    : (size_t)p->tab.ncoef * rs_b;
   }
   fill_ctrl(&p->ctl, cfg);
   if (cfg->contour_points < 2) {
@@ -755,7 +774,15 @@ int rks_coeffs(rks_plan* p, int inst, double h, void* dst, hipStream_t stream) {
   else
     return fail(RKS_ERR_INVALID, "mixed-precision coefficients are generated inside the stepping kernel");
   HIPCHK(hipGetLastError());
-  if (dst) HIPCHK(hipMemcpyAsync(dst, cbase, nbytes, hipMemcpyDeviceToDevice, stream));
+  if (dst && p->coef_real) {
+    // F32 plan with real coefficient storage: ncoef x M floats, returned as complex
+    const size_t n = (size_t)p->tab.ncoef * M;
+    hipLaunchKernelGGL(rks_real_to_complex<float>, dim3((unsigned)((n + 255) / 256 < 4096 ? (n + 255) / 256 : 4096)),
+                       dim3(256), 0, stream, (const float*)cbase, (cx<float>*)dst, n);
+    HIPCHK(hipGetLastError());
+  } else if (dst) {
+    HIPCHK(hipMemcpyAsync(dst, cbase, nbytes, hipMemcpyDeviceToDevice, stream));
+  }
   return RKS_OK;
 }
 

@@ -211,8 +211,11 @@ __device__ __forceinline__ void frac_coefs(TB& tb, int f, const z64 (&v)[4], dou
       else acc = mkc(acc.re / cf.den, acc.im / cf.den);
     }
     char* p = cbase + (size_t)tb.coef_off[c] * M;
-    if (tb.coef_rs[c]) ((cx<RS>*)p)[m] = mkc((RS)acc.re, (RS)acc.im);
-    else ((cx<RN>*)p)[m] = mkc((RN)acc.re, (RN)acc.im);
+    const int kind = tb.coef_rs[c];
+    if (kind == RKS_CST_RS) ((cx<RS>*)p)[m] = mkc((RS)acc.re, (RS)acc.im);
+    else if (kind == RKS_CST_RN) ((cx<RN>*)p)[m] = mkc((RN)acc.re, (RN)acc.im);
+    else if (kind == RKS_CST_RS_REAL) ((RS*)p)[m] = (RS)acc.re;
+    else ((RN*)p)[m] = (RN)acc.re;
   }
 }
 
@@ -370,14 +373,23 @@ template <typename RS, typename RN, int CH>
 __device__ __forceinline__ void load_coef(const rks_tableau& tb, const char* cbase, size_t M, int c,
                                           const ChunkIdx<CH>& ix, cx<RS> (&w)[CH]) {
   const char* q = cbase + (size_t)tb.coef_off[c] * M;
-  if (tb.coef_rs[c]) {
+  const int kind = tb.coef_rs[c];
+  if (kind == RKS_CST_RS) {
     const cx<RS>* p = (const cx<RS>*)q;
 #pragma unroll
     for (int i = 0; i < CH; ++i) w[i] = p[ix.m[i]];
-  } else {
+  } else if (kind == RKS_CST_RN) {
     const cx<RN>* p = (const cx<RN>*)q;
 #pragma unroll
     for (int i = 0; i < CH; ++i) w[i] = cvt<RS>(p[ix.m[i]]);
+  } else if (kind == RKS_CST_RS_REAL) {
+    const RS* p = (const RS*)q;
+#pragma unroll
+    for (int i = 0; i < CH; ++i) w[i] = mkc(p[ix.m[i]], (RS)0);
+  } else {
+    const RN* p = (const RN*)q;
+#pragma unroll
+    for (int i = 0; i < CH; ++i) w[i] = mkc((RS)p[ix.m[i]], (RS)0);
   }
 }
 

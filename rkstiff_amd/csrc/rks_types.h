@@ -26,6 +26,14 @@
 #define RKS_NL1_FSAL 0       // NL1 <- NL(last stage)   (IF34, ETD34, IF45DP, constant methods)
 #define RKS_NL1_RECOMPUTE 1  // NL1 <- nl(u)            (ETD35, etd35.py:317-318)
 
+// coefficient storage kinds (rks_tableau.coef_rs): complex or real (real
+// lin_op in FP32/mixed plans, where every coefficient is real), in NL
+// storage precision RN or state precision RS
+#define RKS_CST_RN 0
+#define RKS_CST_RS 1
+#define RKS_CST_RN_REAL 2
+#define RKS_CST_RS_REAL 3
+
 // instance status
 #define RKS_ST_OK 0
 #define RKS_ST_MAXLOOPS 3
@@ -97,7 +105,8 @@ typedef struct rks_tableau {
   int ncoef;
   int nfrac;
   int uses_contour;    // ETD methods: small modes through the contour mean
-  int coef_rs[RKS_MAX_COEF];   // 1: stored in state precision (multiplies u in some stage)
+  int coef_rs[RKS_MAX_COEF];   // storage of coefficient c (RKS_CST_*); the tableau builder sets
+                               // 1 where it multiplies u in some stage, the plan picks the rest
   int coef_off[RKS_MAX_COEF];  // byte offset of coefficient c per mode (x m), set per plan
   int err_coef;        // factored error estimate: err = C[err_coef] * (((h^err_hpow) * sum(terms)) * err_scale)
   int err_factored;    //   (0: err = sum of the terms)
