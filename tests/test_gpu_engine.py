@@ -92,6 +92,35 @@ def test_coefficients_match_golden(rk, golden, method, lname):
     assert worst < 5e-9
 
 
+@pytest.mark.parametrize("method", ["ETD35", "ETD34", "ETD4", "ETD5", "IF34", "IF4", "IF45DP"])
+def test_real_coefficient_storage_f32(rk, golden, method):
+    """FP32 plans with a built-in term and a real lin_op store every coefficient real;
+    rks_coeffs returns them as complex with zero imaginary parts."""
+    from rkstiff_amd.engine import Plan
+    from tableau_interp import coefficients
+    from rkstiff_amd import _lib
+    g = golden("coeffs")
+    # the golden real lin_op (modes straddling modecutoff) padded to n = 128 points, m = 65 modes
+    L = np.concatenate([np.asarray(g["L/real"]), [-1.0, -10.0, -100.0, -1000.0, 5e-3]])
+    m = L.size
+    n = 2 * (m - 1)
+    assert n == 128
+    kx = np.arange(m, dtype=np.float64)
+    t = _lib.tableau(method)
+    nl = rk.AdvectiveNL(torch.as_tensor(kx), -1.0)
+    plan = Plan(method, nl, _lib.F32, None, None, [dict(n=n, L=L, kx=kx, nl_coef=-1.0)])
+    worst = 0.0
+    for h in g["h"]:
+        got = host(plan.coeffs(0, float(h)))
+        assert np.all(got.imag == 0.0)
+        ref = coefficients(t, L, float(h), solvers.ETDConfig())
+        for c in range(t.ncoef):
+            scale = max(np.max(np.abs(ref[c])), 1e-300)
+            worst = max(worst, float(np.max(np.abs(got[c] - ref[c].real)) / scale))
+    print(f"{method} real storage: max coefficient deviation {worst:.2e}")
+    assert worst < 2e-7
+
+
 # -- whole trajectories vs golden ----------------------------------------------
 def test_ks1024_if34_fp64_matches_reference(rk, golden):
     g = golden("ks1024_if34")
