@@ -263,6 +263,20 @@ struct rks_plan {
 
 static void fill_ctrl(rks_ctrl* c, const rks_config* cfg);
 
+// L[m] == L[(n - m) mod n] bitwise for every mode (L in device memory): the
+// ETD/IF coefficients of mode m depend on h L[m] only, so mirror modes share them.
+static bool mirror_symmetric(const double* L, int cplx, int n) {
+  const int w = cplx ? 2 : 1;
+  std::vector<double> v((size_t)n * w);
+  if (hipMemcpy(v.data(), L, sizeof(double) * v.size(), hipMemcpyDeviceToHost) != hipSuccess) {
+    (void)hipGetLastError();
+    return false;
+  }
+  for (int m = 1; m < n; ++m)
+    if (std::memcmp(&v[(size_t)m * w], &v[(size_t)(n - m) * w], sizeof(double) * w) != 0) return false;
+  return true;
+}
+
 extern "C" {
 
 const char* rks_last_error(void) { return g_err.c_str(); }
@@ -450,6 +464,7 @@ int rks_plan_create(int method, int nl_kind, int precision, const rks_config* cf
     r.ubuf = -1;
     r.rec_pending = -1;
     for (int s = 0; s < RKS_MAX_SLOTS; ++s) r.slot_map[s] = s;
+    if (nl_kind == RKS_NL_CUBIC_FFT) r.coef_sym = mirror_symmetric(r.L, r.L_complex, r.m) ? 1 : 0;
   }
   HIPCHK(hipMemcpy(p->d_inst, p->h_inst.data(), sizeof(rks_inst) * ninst, hipMemcpyHostToDevice));
   if (is2d) {
@@ -608,6 +623,10 @@ int rks_set_storage(rks_plan* p, int store_freq, int snap_cap, int log_cap) {
 }
 
 // start of the coefficient region of instance `inst` (mirror of run_instance)
+// Invalidate an instance's coefficient cache: rks_coeffs overwrites it with
+// all M modes of another step size.
+__global__ void rks_coef_invalidate(rks_inst* in) { in->h_coeff = __builtin_nan(""); }
+
 static char* inst_cbase(const rks_plan* p, int inst) {
   const size_t M = (size_t)p->modes[inst];
   return (char*)p->h_inst[inst].ws + rks_align256((size_t)p->tab.nslot * M * p->nsize) +
@@ -773,6 +792,8 @@ int rks_coeffs(rks_plan* p, int inst, double h, void* dst, hipStream_t stream) {
                        r.L_complex, M, h, cbase);
   else
     return fail(RKS_ERR_INVALID, "mixed-precision coefficients are generated inside the stepping kernel");
+  HIPCHK(hipGetLastError());
+  hipLaunchKernelGGL(rks_coef_invalidate, dim3(1), dim3(1), 0, stream, p->d_inst + inst);
   HIPCHK(hipGetLastError());
   if (dst && p->coef_real) {
     // F32 plan with real coefficient storage: ncoef x M floats, returned as complex

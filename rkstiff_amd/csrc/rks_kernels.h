@@ -310,12 +310,13 @@ __device__ __forceinline__ void coef_small_group(TB& tb, CT& ct, const double* _
 
 // All coefficients of an instance by one workgroup: closed forms per thread,
 // then the small modes' contour means spread over groups of up to 32 lanes
-// (one lane per node, deterministic shuffle reduction).  `list` is LDS
-// scratch for up to M ints; *count is an LDS counter.
+// (one lane per node, deterministic shuffle reduction).  Modes 0..Mg-1 are
+// generated into arrays of stride M.  `list` is LDS scratch for up to Mg
+// ints; *count is an LDS counter.
 template <typename RS, typename RN, int T, int MPT>
 __device__ __attribute__((noinline)) void coef_update(const rks_tableau& tb_in, const rks_ctrl& ct_in,
-                                                      const double* __restrict__ L, int L_complex, int M, double h,
-                                                      char* __restrict__ cbase, int* list, int* count) {
+                                                      const double* __restrict__ L, int L_complex, int M, int Mg,
+                                                      double h, char* __restrict__ cbase, int* list, int* count) {
 #pragma clang fp contract(off)
   rks_ctab& tb = *rks_uniform_const<rks_ctab>(&tb_in);
   rks_cctl& ct = *rks_uniform_const<rks_cctl>(&ct_in);
@@ -325,7 +326,7 @@ __device__ __attribute__((noinline)) void coef_update(const rks_tableau& tb_in, 
   __syncthreads();
   for (int i = 0; i < MPT; ++i) {
     const int m = t + i * T;
-    if (m >= M) break;
+    if (m >= Mg) break;
     const z64 z = mode_z(L, L_complex, m, h);
     if (tb.uses_contour && hypot(z.re, z.im) < ct.modecutoff) {
       list[atomicAdd(count, 1)] = m;
@@ -354,12 +355,20 @@ __device__ __attribute__((noinline)) void coef_update(const rks_tableau& tb_in, 
 // term, and modes past the end load index 0 (always in bounds) instead of
 // branching, so a chunk's loads are straight-line code and issue back to
 // back; callers discard the lanes with ok[i] false.
+//
+// Mirror pairs (sym): slots 2j and 2j+1 hold modes q and n - q (q = 0 pairs
+// with n/2), and the linear operator is mirror-symmetric, so both modes have
+// the same coefficients; they are stored for modes 0..n/2 only and one load
+// serves the pair (n/2 itself is loaded for the q = 0 pair).
 // ---------------------------------------------------------------------------
 template <int CH>
 struct ChunkIdx {
   int m[CH];
   bool ok[CH];
-  __device__ __forceinline__ ChunkIdx(int m0, int T, int mlim) {
+  int sym;   // slots are mirror pairs sharing coefficients
+  int half;  // n/2 (sym only)
+  __device__ __forceinline__ ChunkIdx() {}
+  __device__ __forceinline__ ChunkIdx(int m0, int T, int mlim) : sym(0), half(0) {
 #pragma unroll
     for (int i = 0; i < CH; ++i) {
       const int x = m0 + i * T;
@@ -369,27 +378,86 @@ struct ChunkIdx {
   }
 };
 
+// Mode ownership in the persistent kernel: thread t holds MPT modes in
+// chunks of CH.  Complex-state kernels (M = N) hold mirror pairs, mode
+// q = t + j*T (j < N/(2T)) in slot 2j and its mirror N - q (N/2 for q = 0) in
+// slot 2j + 1; the rfft-layout kernels hold m = t + k*T.  Every M-sized array
+// (state, candidates, NL slots, coefficients) is read and written through
+// this map by the stage passes, so each element a stage pass produces is
+// consumed by the same thread.
+template <int N, int M, int T, int CH>
+__device__ __forceinline__ ChunkIdx<CH> own_chunk(int c, int sym) {
+  ChunkIdx<CH> ix;
+  const int t = threadIdx.x;
+  if constexpr (M == N) {
+    static_assert(CH % 2 == 0 && N % (2 * T) == 0, "whole mirror pairs per chunk");
+    ix.sym = sym;
+    ix.half = N / 2;
+#pragma unroll
+    for (int j = 0; j < CH / 2; ++j) {
+      const int q = t + (c * (CH / 2) + j) * T;
+      ix.m[2 * j] = q;
+      ix.m[2 * j + 1] = q == 0 ? N / 2 : N - q;
+      ix.ok[2 * j] = ix.ok[2 * j + 1] = true;
+    }
+  } else {
+    ix.sym = 0;
+    ix.half = 0;
+#pragma unroll
+    for (int i = 0; i < CH; ++i) {
+      const int x = t + (c * CH + i) * T;
+      ix.ok[i] = x < M;
+      ix.m[i] = ix.ok[i] ? x : 0;
+    }
+  }
+  return ix;
+}
+
+// coefficient array c of the chunk's modes; M is the array stride (modes)
 template <typename RS, typename RN, int CH>
 __device__ __forceinline__ void load_coef(const rks_tableau& tb, const char* cbase, size_t M, int c,
                                           const ChunkIdx<CH>& ix, cx<RS> (&w)[CH]) {
   const char* q = cbase + (size_t)tb.coef_off[c] * M;
   const int kind = tb.coef_rs[c];
+  // Mirror pairs: FP32-state kernels issue one load per pair (the q = 0
+  // pair also loads n/2).  The FP64-state kernels have no registers to spare
+  // for that second code path, so their odd slot repeats the even slot's
+  // address (n/2 for q = 0) and the caches serve the repeat.
+  int ci[CH];
+#pragma unroll
+  for (int i = 0; i < CH; ++i) ci[i] = ix.m[i];
+  if constexpr (CH % 2 == 0 && sizeof(RS) > 4) {
+#pragma unroll
+    for (int j = 0; j < CH / 2; ++j)
+      if (ix.sym) ci[2 * j + 1] = (ix.m[2 * j + 1] == ix.half) ? ix.half : ix.m[2 * j];
+  }
+  auto gather = [&](auto ld) {
+    if constexpr (CH % 2 == 0 && sizeof(RS) == 4) {
+      if (ix.sym) {
+#pragma unroll
+        for (int j = 0; j < CH / 2; ++j) {
+          w[2 * j] = ld(ix.m[2 * j]);
+          w[2 * j + 1] = w[2 * j];
+        }
+        if (ix.m[1] == ix.half) w[1] = ld(ix.half);
+        return;
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < CH; ++i) w[i] = ld(ci[i]);
+  };
   if (kind == RKS_CST_RS) {
     const cx<RS>* p = (const cx<RS>*)q;
-#pragma unroll
-    for (int i = 0; i < CH; ++i) w[i] = p[ix.m[i]];
+    gather([&](int m) { return p[m]; });
   } else if (kind == RKS_CST_RN) {
     const cx<RN>* p = (const cx<RN>*)q;
-#pragma unroll
-    for (int i = 0; i < CH; ++i) w[i] = cvt<RS>(p[ix.m[i]]);
+    gather([&](int m) { return cvt<RS>(p[m]); });
   } else if (kind == RKS_CST_RS_REAL) {
     const RS* p = (const RS*)q;
-#pragma unroll
-    for (int i = 0; i < CH; ++i) w[i] = mkc(p[ix.m[i]], (RS)0);
+    gather([&](int m) { return mkc(p[m], (RS)0); });
   } else {
     const RN* p = (const RN*)q;
-#pragma unroll
-    for (int i = 0; i < CH; ++i) w[i] = mkc((RS)p[ix.m[i]], (RS)0);
+    gather([&](int m) { return mkc((RS)p[m], (RS)0); });
   }
 }
 
@@ -446,10 +514,9 @@ __device__ __forceinline__ void add_terms(const rks_tableau& tb, const rks_term*
 // lanes past mlim come back zero
 template <typename RS, typename RN, int CH, typename SlotF>
 __device__ __forceinline__ void stage_chunk(const rks_tableau& tb, const rks_stage& S, const cx<RS>* __restrict__ u,
-                                            SlotF slotp, const char* cbase, size_t M, int m0, int T, int mlim,
+                                            SlotF slotp, const char* cbase, size_t M, const ChunkIdx<CH>& ix,
                                             double h, cx<RS> (&acc)[CH]) {
   using C = cx<RS>;
-  const ChunkIdx<CH> ix(m0, T, mlim);
 #pragma unroll
   for (int i = 0; i < CH; ++i) acc[i] = u[ix.m[i]];
   if (S.ecoef >= 0) {
@@ -471,11 +538,18 @@ __device__ __forceinline__ void stage_chunk(const rks_tableau& tb, const rks_sta
     if (!ix.ok[i]) acc[i] = mkc((RS)0, (RS)0);
 }
 
+// modes m0 + i*T below mlim
+template <typename RS, typename RN, int CH, typename SlotF>
+__device__ __forceinline__ void stage_chunk(const rks_tableau& tb, const rks_stage& S, const cx<RS>* __restrict__ u,
+                                            SlotF slotp, const char* cbase, size_t M, int m0, int T, int mlim,
+                                            double h, cx<RS> (&acc)[CH]) {
+  stage_chunk<RS, RN, CH>(tb, S, u, slotp, cbase, M, ChunkIdx<CH>(m0, T, mlim), h, acc);
+}
+
 // embedded error estimate (etd35.py:344, etd34.py:190, if34.py:130, if45dp.py:170-177)
 template <typename RS, typename RN, int CH, typename SlotF>
-__device__ __forceinline__ void error_chunk(const rks_tableau& tb, SlotF slotp, const char* cbase, size_t M, int m0,
-                                            int T, int mlim, double h, cx<RS> (&ev)[CH]) {
-  const ChunkIdx<CH> ix(m0, T, mlim);
+__device__ __forceinline__ void error_chunk(const rks_tableau& tb, SlotF slotp, const char* cbase, size_t M,
+                                            const ChunkIdx<CH>& ix, double h, cx<RS> (&ev)[CH]) {
   add_terms<RS, RN, CH, true>(tb, tb.err, tb.nerr, slotp, cbase, M, ix, h, ev);
   if (tb.err_factored) {
     if (tb.err_hpow) {
@@ -498,6 +572,12 @@ __device__ __forceinline__ void error_chunk(const rks_tableau& tb, SlotF slotp, 
     if (!ix.ok[i]) ev[i] = mkc((RS)0, (RS)0);
 }
 
+template <typename RS, typename RN, int CH, typename SlotF>
+__device__ __forceinline__ void error_chunk(const rks_tableau& tb, SlotF slotp, const char* cbase, size_t M, int m0,
+                                            int T, int mlim, double h, cx<RS> (&ev)[CH]) {
+  error_chunk<RS, RN, CH>(tb, slotp, cbase, M, ChunkIdx<CH>(m0, T, mlim), h, ev);
+}
+
 // The last stage and the embedded error read the same NL slots in the same
 // order with scalar error weights (ETD35: k = E u + sum c_j NL_j,
 // err = a75 (-NL1 + 4 NL3 - 6 NL4 + 4 NL5 - NL6), etd35.py:340-344): one pass
@@ -516,11 +596,10 @@ __device__ __forceinline__ bool rks_fused_final(const rks_tableau& tb) {
 
 template <typename RS, typename RN, int CH, typename SlotF>
 __device__ __forceinline__ void final_chunk(const rks_tableau& tb, const cx<RS>* __restrict__ u, SlotF slotp,
-                                            const char* cbase, size_t M, int m0, int T, int mlim, double h,
+                                            const char* cbase, size_t M, const ChunkIdx<CH>& ix, double h,
                                             cx<RS> (&acc)[CH], cx<RS> (&ev)[CH]) {
   using C = cx<RS>;
   const rks_stage& S = tb.st[tb.nstage - 1];
-  const ChunkIdx<CH> ix(m0, T, mlim);
 #pragma unroll
   for (int i = 0; i < CH; ++i) acc[i] = u[ix.m[i]];
   if (S.ecoef >= 0) {
@@ -579,12 +658,12 @@ __device__ __forceinline__ void final_chunk(const rks_tableau& tb, const cx<RS>*
 }
 
 // ---------------------------------------------------------------------------
-// nonlinear term in LDS.  fill_nl writes the spectrum produced by getk(m)
+// nonlinear term in LDS.  fill_nl writes the spectrum produced by getk
 // into the padded LDS buffer (inlined into the stage loop); nl_core runs the
-// transforms and writes NL to dst.  The mode-owner mapping m = tid + i*T is the
-// same for fill and write-back, so every global array element is produced and
-// consumed by one thread.  nl_core is not inlined: its register allocation is
-// independent of the controller and coefficient code around it.
+// transforms and writes NL to dst in the transform's own thread order and
+// ends on a barrier, after which any thread of the workgroup reads NL.
+// nl_core is not inlined: its register allocation is independent of the
+// controller and coefficient code around it.
 // ---------------------------------------------------------------------------
 // modes per chunk of the stage combination
 template <int MPT>
@@ -593,25 +672,24 @@ struct Chunk {
   static constexpr int NCH = (MPT + CH - 1) / CH;
 };
 
-// getk(m0, acc): the CH spectrum values of modes m0 + i*T (i < CH)
+// getk(ix, acc): the CH spectrum values of the chunk's modes ix.m (own_chunk)
 template <typename R, int LOG2N, int T, int NLK, typename GetK>
-__device__ __forceinline__ void fill_nl(GetK getk, cx<R>* buf, const R* __restrict__ kx) {
+__device__ __forceinline__ void fill_nl(GetK getk, cx<R>* buf, const R* __restrict__ kx, int sym) {
   using C = cx<R>;
   constexpr int N = 1 << LOG2N;
   constexpr int M = (NLK == NLK_ADVECT) ? N / 2 + 1 : N;
   constexpr int MPT = (M + T - 1) / T;
   constexpr int CH = Chunk<MPT>::CH;
   constexpr int H = N / 2;
-  const int t = threadIdx.x;
 #pragma unroll 1
   for (int c = 0; c < Chunk<MPT>::NCH; ++c) {
-    const int m0 = t + c * CH * T;
+    const ChunkIdx<CH> ix = own_chunk<N, M, T, CH>(c, sym);
     C acc[CH];
-    getk(m0, acc);
+    getk(ix, acc);
 #pragma unroll
     for (int i = 0; i < CH; ++i) {
-      const int m = m0 + i * T;
-      if (m >= M) continue;
+      if (!ix.ok[i]) continue;
+      const int m = ix.m[i];
       if constexpr (NLK == NLK_CUBIC) {
         buf[rksfft::pad(m)] = acc[i];
       } else {
@@ -686,8 +764,8 @@ __device__ __attribute__((noinline)) void nl_core(cx<R>* buf, const cx<R>* twl, 
 
 template <typename R, typename RN, int LOG2N, int T, int NLK, typename GetK>
 __device__ __forceinline__ void eval_nl(GetK getk, cx<R>* buf, const cx<R>* twl, const R* __restrict__ kx, R coef,
-                                        cx<RN>* __restrict__ dst) {
-  fill_nl<R, LOG2N, T, NLK>(getk, buf, kx);
+                                        cx<RN>* __restrict__ dst, int sym) {
+  fill_nl<R, LOG2N, T, NLK>(getk, buf, kx, sym);
   nl_core<R, RN, LOG2N, T, NLK>(buf, twl, coef, dst);
 }
 
@@ -788,6 +866,7 @@ __device__ void run_instance(const rks_launch& P, int idx, cx<R>* buf, const cx<
   constexpr int N = 1 << LOG2N;
   constexpr int M = (NLK == NLK_ADVECT) ? N / 2 + 1 : N;
   constexpr int MPT = (M + T - 1) / T;
+  constexpr int CH = Chunk<MPT>::CH;
   const rks_tableau* __restrict__ tbp = P.tab;
   const rks_ctrl* __restrict__ ctp = P.ctl;
   const rks_tableau& tb = *tbp;
@@ -807,17 +886,20 @@ __device__ void run_instance(const rks_launch& P, int idx, cx<R>* buf, const cx<
   C* const cand0 = (C*)(ws + rks_align256((size_t)nslot * M * sizeof(CN)));
   C* const cand1 = cand0 + rks_align256((size_t)M * sizeof(C)) / sizeof(C);
   char* cbase = (char*)(cand1 + rks_align256((size_t)M * sizeof(C)) / sizeof(C));
+  // mirror-symmetric L (complex state): coefficients of modes 0..N/2 in
+  // arrays of stride Mc (a multiple of 64 modes, at most M)
+  const int sym = (M == N) && in->coef_sym;
+  constexpr int MSYM = ((N / 2 + 1 + 63) / 64) * 64 < M ? ((N / 2 + 1 + 63) / 64) * 64 : M;
+  const int Mc = sym ? MSYM : M;
+  const int Mg = sym ? N / 2 + 1 : M;
 
   if (P.mode == RKS_MODE_NL) {
     const C* u = (const C*)io.u_in;
-    auto getu = [&](int m0, C(&acc)[Chunk<MPT>::CH]) {
+    auto getu = [&](const ChunkIdx<CH>& ix, C(&acc)[CH]) {
 #pragma unroll
-      for (int i = 0; i < Chunk<MPT>::CH; ++i) {
-        const int m = m0 + i * T;
-        acc[i] = m < M ? u[m] : mkc((R)0, (R)0);
-      }
+      for (int i = 0; i < CH; ++i) acc[i] = ix.ok[i] ? u[ix.m[i]] : mkc((R)0, (R)0);
     };
-    eval_nl<R, R, LOG2N, T, NLK>(getu, buf, twl, kx, nlc, (C*)io.u_out);
+    eval_nl<R, R, LOG2N, T, NLK>(getu, buf, twl, kx, nlc, (C*)io.u_out, 0);
     return;
   }
   if (P.mode == RKS_MODE_COEFFS) {
@@ -909,24 +991,21 @@ __device__ void run_instance(const rks_launch& P, int idx, cx<R>* buf, const cx<
     // ---------------- attempt loop (solveras.py:384-410) ----------------
     for (;;) {
       if (h != h_coeff) {
-        coef_update<R, RN, T, MPT>(tb, ct, Ld, Lc, M, h, cbase, (int*)buf, lds_count);
+        coef_update<R, RN, T, MPT>(tb, ct, Ld, Lc, Mc, Mg, h, cbase, (int*)buf, lds_count);
         h_coeff = h;
         ++coef_updates;
       }
-      auto getu = [&](int m0, C(&acc)[Chunk<MPT>::CH]) {
+      auto getu = [&](const ChunkIdx<CH>& ix, C(&acc)[CH]) {
 #pragma unroll
-        for (int i = 0; i < Chunk<MPT>::CH; ++i) {
-          const int m = m0 + i * T;
-          acc[i] = m < M ? u[m] : mkc((R)0, (R)0);
-        }
+        for (int i = 0; i < CH; ++i) acc[i] = ix.ok[i] ? u[ix.m[i]] : mkc((R)0, (R)0);
       };
       if (!started) {
-        eval_nl<R, RN, LOG2N, T, NLK>(getu, buf, twl, kx, nlc, slot(0));
+        eval_nl<R, RN, LOG2N, T, NLK>(getu, buf, twl, kx, nlc, slot(0), sym);
         ++nl_calls;
         started = 1;
       } else if (accept || !tb.adaptive) {
         if (tb.nl1_policy == RKS_NL1_RECOMPUTE) {
-          eval_nl<R, RN, LOG2N, T, NLK>(getu, buf, twl, kx, nlc, slot(0));
+          eval_nl<R, RN, LOG2N, T, NLK>(getu, buf, twl, kx, nlc, slot(0), sym);
           ++nl_calls;
         } else {
           const unsigned f4 = 4u * (unsigned)tb.fsal_slot;
@@ -944,14 +1023,13 @@ __device__ void run_instance(const rks_launch& P, int idx, cx<R>* buf, const cx<
           __syncthreads();  // the previous stage's write-back still reads buf
 #pragma unroll 1
           for (int c = 0; c < Chunk<MPT>::NCH; ++c) {
-            constexpr int CH = Chunk<MPT>::CH;
-            const int m0 = t + c * CH * T;
+            const ChunkIdx<CH> ix = own_chunk<N, M, T, CH>(c, sym);
             C acc[CH], ev[CH];
-            final_chunk<R, RN, CH>(tb, u, slot, cbase, (size_t)M, m0, T, M, h, acc, ev);
+            final_chunk<R, RN, CH>(tb, u, slot, cbase, (size_t)Mc, ix, h, acc, ev);
 #pragma unroll
             for (int i = 0; i < CH; ++i) {
-              const int m = m0 + i * T;
-              if (m < M) {
+              const int m = ix.m[i];
+              if (ix.ok[i]) {
                 cand[m] = acc[i];
                 const double a = hypot((double)acc[i].re, (double)acc[i].im);
                 const double er = (double)ev[i].re, ei = (double)ev[i].im;
@@ -962,25 +1040,25 @@ __device__ void run_instance(const rks_launch& P, int idx, cx<R>* buf, const cx<
           }
           continue;
         }
-        auto getk = [&](int m0, C(&acc)[Chunk<MPT>::CH]) {
-          stage_chunk<R, RN, Chunk<MPT>::CH>(tb, S, u, slot, cbase, (size_t)M, m0, T, M, h, acc);
+        auto getk = [&](const ChunkIdx<CH>& ix, C(&acc)[CH]) {
+          stage_chunk<R, RN, CH>(tb, S, u, slot, cbase, (size_t)Mc, ix, h, acc);
           if (last) {
 #pragma unroll
-            for (int i = 0; i < Chunk<MPT>::CH; ++i)
-              if (m0 + i * T < M) {
-                cand[m0 + i * T] = acc[i];
+            for (int i = 0; i < CH; ++i)
+              if (ix.ok[i]) {
+                cand[ix.m[i]] = acc[i];
                 if (tb.adaptive) mx = nanmax(mx, hypot((double)acc[i].re, (double)acc[i].im));
               }
           }
         };
         if (S.has_nl) {
-          eval_nl<R, RN, LOG2N, T, NLK>(getk, buf, twl, kx, nlc, slot(S.nl_slot));
+          eval_nl<R, RN, LOG2N, T, NLK>(getk, buf, twl, kx, nlc, slot(S.nl_slot), sym);
           ++nl_calls;
         } else {
 #pragma unroll 1
           for (int c = 0; c < Chunk<MPT>::NCH; ++c) {
-            C acc[Chunk<MPT>::CH];
-            getk(t + c * Chunk<MPT>::CH * T, acc);
+            C acc[CH];
+            getk(own_chunk<N, M, T, CH>(c, sym), acc);
           }
         }
       }
@@ -993,11 +1071,15 @@ __device__ void run_instance(const rks_launch& P, int idx, cx<R>* buf, const cx<
       // max |u_new| (taken above); masked sums of |u_new|^2 and |err|^2 (FP64)
       mx = wg_max<T>(mx, red);
       double su = 0.0, se = 0.0;
+      // each thread reads back the modes it wrote (own_chunk)
       if (fused) {
-        for (int i = 0; i < MPT; ++i) {
-          const int m = t + i * T;
-          if (m < M) {
-            const C v = buf[m];
+#pragma unroll 1
+        for (int c = 0; c < Chunk<MPT>::NCH; ++c) {
+          const ChunkIdx<CH> ix = own_chunk<N, M, T, CH>(c, sym);
+#pragma unroll
+          for (int i = 0; i < CH; ++i) {
+            if (!ix.ok[i]) continue;
+            const C v = buf[ix.m[i]];
             const double a = (double)v.re;
             if (a / mx > ct.adapt_cutoff) {
               su += a * a;
@@ -1008,14 +1090,13 @@ __device__ void run_instance(const rks_launch& P, int idx, cx<R>* buf, const cx<
       } else {
 #pragma unroll 1
       for (int c = 0; c < Chunk<MPT>::NCH; ++c) {
-        constexpr int CH = Chunk<MPT>::CH;
-        const int m0 = t + c * CH * T;
+        const ChunkIdx<CH> ix = own_chunk<N, M, T, CH>(c, sym);
         C ev[CH];
-        error_chunk<R, RN, CH>(tb, slot, cbase, (size_t)M, m0, T, M, h, ev);
+        error_chunk<R, RN, CH>(tb, slot, cbase, (size_t)Mc, ix, h, ev);
 #pragma unroll
         for (int i = 0; i < CH; ++i) {
-          const int m = m0 + i * T;
-          if (m < M) {
+          const int m = ix.m[i];
+          if (ix.ok[i]) {
             const C kv = cand[m];
             const double kr = (double)kv.re, ki = (double)kv.im;
             if (hypot(kr, ki) / mx > ct.adapt_cutoff) {
@@ -1112,9 +1193,11 @@ __device__ void run_instance(const rks_launch& P, int idx, cx<R>* buf, const cx<
     if (store_freq > 0 && (steps % store_freq) == 0) {
       C* dst = (C*)in->snap + (size_t)snap_count * M;
       const C* src = cur_u();
-      for (int i = 0; i < MPT; ++i) {
-        const int m = t + i * T;
-        if (m < M) dst[m] = src[m];
+      for (int c = 0; c < Chunk<MPT>::NCH; ++c) {
+        const ChunkIdx<CH> ix = own_chunk<N, M, T, CH>(c, sym);
+#pragma unroll
+        for (int i = 0; i < CH; ++i)
+          if (ix.ok[i]) dst[ix.m[i]] = src[ix.m[i]];
       }
       if (t == 0) in->snap_t[snap_count] = tc;
       ++snap_count;
@@ -1123,9 +1206,11 @@ __device__ void run_instance(const rks_launch& P, int idx, cx<R>* buf, const cx<
   if (evolve && status != RKS_ST_PAUSED) {
     const C* src = cur_u();
     C* dst = (C*)io.u_out;
-    for (int i = 0; i < MPT; ++i) {
-      const int m = t + i * T;
-      if (m < M) dst[m] = src[m];
+    for (int c = 0; c < Chunk<MPT>::NCH; ++c) {
+      const ChunkIdx<CH> ix = own_chunk<N, M, T, CH>(c, sym);
+#pragma unroll
+      for (int i = 0; i < CH; ++i)
+        if (ix.ok[i]) dst[ix.m[i]] = src[ix.m[i]];
     }
   }
   __syncthreads();

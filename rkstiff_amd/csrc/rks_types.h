@@ -160,7 +160,7 @@ typedef struct rks_inst {
   float* rec;
   int rec_cap, rec_count;
   int rec_pending;     // record whose target the next attempt fills (-1: none)
-  int pad_rec;
+  int coef_sym;        // L[m] == L[n - m] bitwise (complex state): coefficients kept for modes 0..n/2
   double h_prop;       // step proposed after the last accept, before the tf clamp
 } rks_inst;
 

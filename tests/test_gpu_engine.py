@@ -243,6 +243,56 @@ def test_nls256_engine_fp64(rk, golden, method):
     assert err < 1e-10
 
 
+def _nls_drift(n=256, drift=0.4):
+    """Cubic NLS on [-32, 32) with a drift term, L = i(-k^2/2 + drift k): L[m] != L[n - m]."""
+    x = -32.0 + 64.0 * np.arange(n) / n
+    k = 2.0 * np.pi * np.fft.fftfreq(n, d=64.0 / n)
+    u0 = np.fft.fft(1.2 / np.cosh(1.2 * x) * np.exp(0.3j * x))
+    return 1j * (-0.5 * k**2 + drift * k), u0
+
+
+@pytest.mark.parametrize("method", ADAPTIVE + CONSTANT)
+def test_cubic_engine_asymmetric_L(rk, method):
+    """Mirror modes share coefficients only when L[m] == L[n - m]; a drift term breaks that, and
+    every mode's coefficients are then kept.  Engine vs the oracle on the drifting soliton."""
+    L, u0 = _nls_drift()
+    cfg = rk.SolverConfig(epsilon=1e-5)
+    ocfg = solvers.SolverConfig(epsilon=1e-5)
+    cls = getattr(rk, method)
+    nl = rk.CubicNL(256, 1.0)
+    drv = solvers.make_solver(method, L, problems.nls_nl(1.0), ocfg)
+    if method in ADAPTIVE:
+        uf = cls(torch.as_tensor(L), nl, config=cfg).evolve(dev(u0), t0=0.0, tf=2.0, store_data=False)
+        ref = drv.evolve(u0, t0=0.0, tf=2.0, store_data=False)
+    else:
+        uf = cls(torch.as_tensor(L), nl).evolve(dev(u0), t0=0.0, tf=2.0, h=0.01, store_data=False)
+        ref = drv.evolve(u0, t0=0.0, tf=2.0, h=0.01, store_data=False)
+    err = rel(host(uf), ref)
+    print(method, "asymmetric L engine vs oracle", err)
+    assert err < 1e-10
+
+
+@pytest.mark.parametrize("precision,tol", [("f64", 1e-10), ("mixed", 1e-5), ("f32", 1e-3)])
+def test_batch_symmetric_and_asymmetric_L(rk, precision, tol):
+    """One launch with a mirror-symmetric L (shared coefficients) and a drifting one (full
+    coefficients): each instance against its own oracle run (FP64) or the FP64 engine."""
+    from rkstiff_amd.batch import BatchSolver
+    cases = [_nls_drift(drift=0.0), _nls_drift(drift=0.4), _nls_drift(drift=0.0)]
+    assert np.array_equal(cases[0][0][1:], cases[0][0][1:][::-1])
+    probs = [(torch.as_tensor(L), rk.CubicNL(256, 1.0)) for L, _ in cases]
+    cfg = rk.SolverConfig(epsilon=1e-5)
+    dt = torch.complex64 if precision == "f32" else torch.complex128
+    bs = BatchSolver("ETD35", probs, precision=precision, config=cfg)
+    uf = bs.evolve([dev(u0, dt) for _, u0 in cases], 0.0, 2.0)
+    assert not bs.failures()
+    for (L, u0), u in zip(cases, uf):
+        drv = solvers.make_solver("ETD35", L, problems.nls_nl(1.0), solvers.SolverConfig(epsilon=1e-5))
+        ref = drv.evolve(u0, t0=0.0, tf=2.0, store_data=False)
+        err = rel(host(u).astype(np.complex128), ref)
+        print(precision, "batch vs oracle", err)
+        assert err < tol
+
+
 @pytest.mark.parametrize("method", ["ETD35", "IF34"])
 def test_kdv_fp32_close_to_fp64(rk, golden, method):
     g = golden("kdv_burgers")
