@@ -65,7 +65,10 @@ struct Geo2 {
   static constexpr int CTG = (N / 2 < 512 / CG) ? N / 2 : 512 / CG;
   static constexpr int CT = CG * CTG;
   static constexpr int CEPT = N / CTG;  // tile elements per thread
-  static constexpr int CCH = CEPT < 8 ? CEPT : 8;
+  // modes per chunk of the stage combination: 4 in FP64 keeps the column
+  // kernel within 128 VGPRs without spills (two workgroups per CU, LDS-bound)
+  static constexpr int CCH_MAX = sizeof(R) == 4 ? 8 : 4;
+  static constexpr int CCH = CEPT < CCH_MAX ? CEPT : CCH_MAX;
   static constexpr int NCG = (NXH + CG - 1) / CG;
   static constexpr int COL_LDS = (CG * PS + TWS) * (int)sizeof(cx<R>);
   // row kernel: RG complex transforms (2 RG rows), RTG threads each
@@ -112,8 +115,10 @@ __device__ __forceinline__ void rks2_states(const rks2_launch& P, const rks_inst
 // trajectories that evaluated NL1 in this attempt; prelude: the whole kernel
 // only for those.
 // ---------------------------------------------------------------------------
+// Launch bounds ask for 4 waves per SIMD (<= 128 VGPRs): with ~70 KB of LDS
+// per workgroup at N = 512, two workgroups share a CU.
 template <typename R, typename RN, int LOG2>
-__global__ __launch_bounds__((Geo2<R, LOG2>::CT)) void k2_col(rks2_launch P, int fwd_slot, int fwd_need_nl1,
+__global__ __launch_bounds__((Geo2<R, LOG2>::CT), 4) void k2_col(rks2_launch P, int fwd_slot, int fwd_need_nl1,
                                                             int arg, int prelude) {
   using G = Geo2<R, LOG2>;
   using C = cx<R>;
