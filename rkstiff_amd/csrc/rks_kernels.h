@@ -1237,10 +1237,14 @@ __device__ void run_instance(const rks_launch& P, int idx, cx<R>* buf, const cx<
   }
 }
 
-// waves per SIMD the register allocation must allow: the FFT core fits in
-// 256 (n = 8192, 512 threads) or 168 registers; the rarely executed FP64
-// coefficient code spills instead of lowering occupancy.
-__host__ __device__ constexpr int rks_min_waves(int log2n) { return log2n >= 13 ? 2 : 3; }
+// waves per SIMD the register allocation must allow.  FP32-state kernels
+// below n = 8192: 3 (168 registers; the rarely executed FP64 coefficient
+// code spills instead of lowering occupancy).  FP64-state kernels and
+// n = 8192: 2 (256 registers): at 168 their stage passes spill too, and the
+// KS ensemble ran 5-19% slower in FP64 and 9-14% in MIXED than at 256.
+__host__ __device__ constexpr int rks_min_waves(int log2n, int state_bytes) {
+  return (log2n >= 13 || state_bytes == 8) ? 2 : 3;
+}
 
 // LEARN: the variant with the learned step-size proposer and its training
 // records; the plain variant carries none of that code, so the stepping loop
@@ -1248,7 +1252,8 @@ __host__ __device__ constexpr int rks_min_waves(int log2n) { return log2n >= 13 
 // TILE 1: twice the threads per trajectory (half the modes per thread in the
 // stage passes and FFTs), where every transform keeps >= 2 elements per thread.
 template <typename R, typename RN, int LOG2N, int NLK, bool LEARN, int TILE>
-__global__ __launch_bounds__(rks_threads(LOG2N, NLK) << TILE, rks_min_waves(LOG2N)) void rks_kernel(rks_launch P) {
+__global__ __launch_bounds__(rks_threads(LOG2N, NLK) << TILE, rks_min_waves(LOG2N, (int)sizeof(R))) void rks_kernel(
+    rks_launch P) {
   constexpr int T = rks_threads(LOG2N, NLK) << TILE;
   static_assert(T <= 512 && T <= (NLK == NLK_ADVECT ? (1 << LOG2N) / 4 : (1 << LOG2N) / 2), "tile too wide");
   extern __shared__ __align__(16) unsigned char rks_smem[];

@@ -272,6 +272,30 @@ def test_cubic_engine_asymmetric_L(rk, method):
     assert err < 1e-10
 
 
+def test_coefficient_copy_out_keeps_stepping_cache_consistent(rk):
+    """With a mirror-symmetric lin_op the stepping kernel caches coefficients of modes 0..n/2;
+    rks_coeffs overwrites that cache with all n modes at another h.  The copy-out must match the
+    tableau interpreter on every mode, and the next step at the cached h must regenerate rather
+    than read the overwritten arrays (bit-identical to the step before the copy-out)."""
+    from rkstiff_amd.engine import Plan
+    from rkstiff_amd import _lib
+    from tableau_interp import coefficients
+    L, u0 = _nls_drift(drift=0.0)
+    nl = rk.CubicNL(256, 1.0)
+    plan = Plan("ETD35", nl, _lib.F64, rk.SolverConfig(epsilon=1e-5), None, [nl.descriptor(L)])
+    u = dev(u0)
+    a, b = torch.empty_like(u), torch.empty_like(u)
+    plan.step([u], [a], [1e-3])
+    got = host(plan.coeffs(0, 0.037))
+    ref = coefficients(_lib.tableau("ETD35"), L, 0.037, solvers.ETDConfig())
+    for c in range(len(ref)):
+        scale = max(np.max(np.abs(ref[c])), 1e-300)
+        assert np.max(np.abs(got[c] - ref[c])) / scale < 5e-9, c
+    plan.step([u], [b], [1e-3])
+    assert plan.stats()[0]["attempts"] == 1
+    assert torch.equal(a, b)
+
+
 @pytest.mark.parametrize("precision,tol", [("f64", 1e-10), ("mixed", 1e-5), ("f32", 1e-3)])
 def test_batch_symmetric_and_asymmetric_L(rk, precision, tol):
     """One launch with a mirror-symmetric L (shared coefficients) and a drifting one (full
