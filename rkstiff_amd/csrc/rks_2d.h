@@ -78,6 +78,10 @@ struct Geo2 {
   static constexpr int NRG = N / (2 * RG);
   static constexpr int ROW_LDS = (RG * PS + TWS) * (int)sizeof(cx<R>);
   // norm kernel
+  // mirror-symmetric lin_op (L[r][c] == L[n - r][c]): coefficients of rows
+  // 0..n/2 in arrays of stride MSYM (a multiple of 64 modes)
+  static constexpr int MGEN_SYM = (N / 2 + 1) * NXH;
+  static constexpr int MSYM = ((MGEN_SYM + 63) / 64) * 64 < M ? ((MGEN_SYM + 63) / 64) * 64 : M;
   static constexpr int NT = 256, NE = 8;
   static constexpr int NNB = (M + NT * NE - 1) / (NT * NE);
   static_assert(N >= 2 * RG, "grid too small for the row kernel");
@@ -183,25 +187,41 @@ __global__ __launch_bounds__((Geo2<R, LOG2>::CT), 4) void k2_col(rks2_launch P, 
   if (arg < -1) return;
   const bool has_nl = arg < 0 || tb.st[arg].has_nl;
   const bool last = arg >= 0 && arg == tb.nstage - 1;
+  const int sym = in->coef_sym;
+  const size_t Mc = sym ? G::MSYM : M;
   double mx = 0.0;
+  // stage combination over mirror row pairs: slot 2j row q = r0 + (ch CH/2 + j) RS,
+  // slot 2j + 1 row n - q (n/2 for q = 0), so a mirror-symmetric L shares coefficients
 #pragma unroll 1
   for (int ch = 0; ch < NCH; ++ch) {
-    const int rr = r0 + ch * CH * RS;
-    const int m0 = rr * NXH + col;
+    ChunkIdx<CH> ix;
+    int row[CH];
+    ix.sym = sym;
+    ix.lim = G::MGEN_SYM;
+    ix.anyq = 0;
+#pragma unroll
+    for (int j = 0; j < CH / 2; ++j) {
+      const int q = r0 + (ch * (CH / 2) + j) * RS;
+      row[2 * j] = q;
+      row[2 * j + 1] = q == 0 ? N / 2 : N - q;
+      ix.m[2 * j] = valid ? row[2 * j] * NXH + col : 0;
+      ix.m[2 * j + 1] = valid ? row[2 * j + 1] * NXH + col : 0;
+      ix.ok[2 * j] = ix.ok[2 * j + 1] = valid;
+    }
     C acc[CH];
     if (arg < 0) {
 #pragma unroll
-      for (int i = 0; i < CH; ++i) acc[i] = valid ? u[m0 + i * RS * NXH] : zero;
+      for (int i = 0; i < CH; ++i) acc[i] = valid ? u[ix.m[i]] : zero;
     } else {
-      stage_chunk<R, RN, CH>(tb, tb.st[arg], u, slot, cbase, (size_t)M, m0, RS * NXH, valid ? M : 0, h, acc);
+      stage_chunk<R, RN, CH>(tb, tb.st[arg], u, slot, cbase, Mc, ix, h, acc);
     }
 #pragma unroll
     for (int i = 0; i < CH; ++i) {
       if (last && valid) {
-        cand[m0 + i * RS * NXH] = acc[i];
+        cand[ix.m[i]] = acc[i];
         mx = nanmax(mx, hypot((double)acc[i].re, (double)acc[i].im));
       }
-      if (has_nl) bc[pad(rr + i * RS)] = valid ? acc[i] : zero;
+      if (has_nl) bc[pad(row[i])] = valid ? acc[i] : zero;
     }
   }
   if (last && tb.adaptive) {
@@ -285,15 +305,19 @@ __global__ __launch_bounds__((Geo2<R, LOG2>::RT)) void k2_row(rks2_launch P, int
 // ---------------------------------------------------------------------------
 // coefficients (FP64 arithmetic) of trajectories whose h changed
 // ---------------------------------------------------------------------------
-template <typename R, typename RN>
-__global__ __launch_bounds__(256) void k2_coef(rks2_launch P, int M) {
+template <typename R, typename RN, int LOG2>
+__global__ __launch_bounds__(256) void k2_coef(rks2_launch P) {
+  using G = Geo2<R, LOG2>;
   const int inst = blockIdx.y;
   const rks2_state st = P.st[inst];
   if (!st.active || !st.need_coef) return;
   const rks_inst* in = P.inst + inst;
   char* cbase = (char*)in->ws + P.off_coef;
-  for (int m = blockIdx.x * 256 + threadIdx.x; m < M; m += gridDim.x * 256)
-    coef_mode<R, RN>(*P.tab, *P.ctl, in->L, in->L_complex, m, M, in->h, cbase);
+  // mirror-symmetric L: rows 0..n/2 only (stride MSYM)
+  const int Mg = in->coef_sym ? G::MGEN_SYM : G::M;
+  const int Mc = in->coef_sym ? G::MSYM : G::M;
+  for (int m = blockIdx.x * 256 + threadIdx.x; m < Mg; m += gridDim.x * 256)
+    coef_mode<R, RN>(*P.tab, *P.ctl, in->L, in->L_complex, m, Mc, in->h, cbase);
 }
 
 
@@ -324,16 +348,30 @@ __global__ __launch_bounds__(256) void k2_norm(rks2_launch P) {
   double mx = pm[0];
   for (int j = 1; j < P.ncg; ++j) mx = nanmax(mx, pm[j]);
   const int t = threadIdx.x;
-  const int m0 = blockIdx.x * NT * NE + t;
+  // NE/2 mirror row pairs per thread: pair p = (q, col) holds rows q and n - q (n/2 for q = 0)
+  constexpr int N = G::N, NXH = G::NXH;
+  const int sym = in->coef_sym;
+  ChunkIdx<NE> ix;
+  ix.sym = sym;
+  ix.lim = G::MGEN_SYM;
+  ix.anyq = 1;
+#pragma unroll
+  for (int j = 0; j < NE / 2; ++j) {
+    const int p = blockIdx.x * NT * (NE / 2) + t + j * NT;
+    const bool ok = p < (N / 2) * NXH;
+    const int q = ok ? p / NXH : 0, col = ok ? p % NXH : 0;
+    ix.m[2 * j] = q * NXH + col;
+    ix.m[2 * j + 1] = ok ? (q == 0 ? N / 2 : N - q) * NXH + col : 0;
+    ix.ok[2 * j] = ix.ok[2 * j + 1] = ok;
+  }
   C ev[NE];
-  error_chunk<R, RN, NE>(tb, slot, ws + P.off_coef, (size_t)M, m0, NT, M, in->h, ev);
+  error_chunk<R, RN, NE>(tb, slot, ws + P.off_coef, sym ? (size_t)G::MSYM : (size_t)M, ix, in->h, ev);
   const double cut = P.ctl->adapt_cutoff;
   double su = 0.0, se = 0.0;
 #pragma unroll
   for (int i = 0; i < NE; ++i) {
-    const int m = m0 + i * NT;
-    if (m < M) {
-      const C kv = cand[m];
+    if (ix.ok[i]) {
+      const C kv = cand[ix.m[i]];
       const double kr = (double)kv.re, ki = (double)kv.im;
       if (hypot(kr, ki) / mx > cut) {
         const double er = (double)ev[i].re, ei = (double)ev[i].im;

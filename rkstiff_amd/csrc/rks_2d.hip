@@ -49,8 +49,7 @@ struct Eng2 {
   // one attempt of every running trajectory
   static void attempt(const rks2_launch& L, const rks_tableau& tb, hipStream_t s) {
     hipLaunchKernelGGL(k2_begin, dim3(1), dim3(256), 0, s, L);
-    hipLaunchKernelGGL((k2_coef<R, RN>), dim3((G::M + 256 * 8 - 1) / (256 * 8), L.ninst), dim3(256), 0, s, L,
-                       G::M);
+    hipLaunchKernelGGL((k2_coef<R, RN, LOG2>), dim3((G::M + 256 * 8 - 1) / (256 * 8), L.ninst), dim3(256), 0, s, L);
     // NL1 = nl(u) where needed (start; after an accept for RECOMPUTE methods)
     col(L, s, -1, 0, -1, 1);
     row(L, s, 1);

@@ -263,17 +263,19 @@ struct rks_plan {
 
 static void fill_ctrl(rks_ctrl* c, const rks_config* cfg);
 
-// L[m] == L[(n - m) mod n] bitwise for every mode (L in device memory): the
-// ETD/IF coefficients of mode m depend on h L[m] only, so mirror modes share them.
-static bool mirror_symmetric(const double* L, int cplx, int n) {
+// L[r][c] == L[(rows - r) mod rows][c] bitwise for every mode of a rows x cols
+// lin_op in device memory (1-D: cols = 1): the ETD/IF coefficients of a mode
+// depend on h L only, so mirror modes share them.
+static bool mirror_symmetric(const double* L, int cplx, int rows, int cols) {
   const int w = cplx ? 2 : 1;
-  std::vector<double> v((size_t)n * w);
+  const size_t rw = (size_t)cols * w;
+  std::vector<double> v((size_t)rows * rw);
   if (hipMemcpy(v.data(), L, sizeof(double) * v.size(), hipMemcpyDeviceToHost) != hipSuccess) {
     (void)hipGetLastError();
     return false;
   }
-  for (int m = 1; m < n; ++m)
-    if (std::memcmp(&v[(size_t)m * w], &v[(size_t)(n - m) * w], sizeof(double) * w) != 0) return false;
+  for (int r = 1; r < rows; ++r)
+    if (std::memcmp(&v[(size_t)r * rw], &v[(size_t)(rows - r) * rw], sizeof(double) * rw) != 0) return false;
   return true;
 }
 
@@ -464,7 +466,8 @@ int rks_plan_create(int method, int nl_kind, int precision, const rks_config* cf
     r.ubuf = -1;
     r.rec_pending = -1;
     for (int s = 0; s < RKS_MAX_SLOTS; ++s) r.slot_map[s] = s;
-    if (nl_kind == RKS_NL_CUBIC_FFT) r.coef_sym = mirror_symmetric(r.L, r.L_complex, r.m) ? 1 : 0;
+    if (nl_kind == RKS_NL_CUBIC_FFT) r.coef_sym = mirror_symmetric(r.L, r.L_complex, r.m, 1) ? 1 : 0;
+    if (is2d) r.coef_sym = mirror_symmetric(r.L, r.L_complex, r.n, r.m / r.n) ? 1 : 0;
   }
   HIPCHK(hipMemcpy(p->d_inst, p->h_inst.data(), sizeof(rks_inst) * ninst, hipMemcpyHostToDevice));
   if (is2d) {

@@ -356,19 +356,23 @@ __device__ __attribute__((noinline)) void coef_update(const rks_tableau& tb_in, 
 // branching, so a chunk's loads are straight-line code and issue back to
 // back; callers discard the lanes with ok[i] false.
 //
-// Mirror pairs (sym): slots 2j and 2j+1 hold modes q and n - q (q = 0 pairs
-// with n/2), and the linear operator is mirror-symmetric, so both modes have
-// the same coefficients; they are stored for modes 0..n/2 only and one load
-// serves the pair (n/2 itself is loaded for the q = 0 pair).
+// Mirror pairs (sym): slots 2j and 2j+1 hold a mode and its mirror (1-D: q
+// and n - q; 2-D: rows q and n - q of one column; q = 0 pairs with n/2), and
+// the linear operator is mirror-symmetric, so both have the same
+// coefficients.  Coefficients are stored for the modes below `lim` only (1-D:
+// 0..n/2; 2-D: rows 0..n/2); an odd slot at or past lim reads its even
+// slot's coefficient, one below lim (the n/2 partner of q = 0) its own.
+// anyq = 0: only slot 1 can hold such a partner.
 // ---------------------------------------------------------------------------
 template <int CH>
 struct ChunkIdx {
   int m[CH];
   bool ok[CH];
   int sym;   // slots are mirror pairs sharing coefficients
-  int half;  // n/2 (sym only)
+  int lim;   // first mode without stored coefficients (sym only)
+  int anyq;  // any odd slot may lie below lim (else only slot 1)
   __device__ __forceinline__ ChunkIdx() {}
-  __device__ __forceinline__ ChunkIdx(int m0, int T, int mlim) : sym(0), half(0) {
+  __device__ __forceinline__ ChunkIdx(int m0, int T, int mlim) : sym(0), lim(0), anyq(0) {
 #pragma unroll
     for (int i = 0; i < CH; ++i) {
       const int x = m0 + i * T;
@@ -392,7 +396,8 @@ __device__ __forceinline__ ChunkIdx<CH> own_chunk(int c, int sym) {
   if constexpr (M == N) {
     static_assert(CH % 2 == 0 && N % (2 * T) == 0, "whole mirror pairs per chunk");
     ix.sym = sym;
-    ix.half = N / 2;
+    ix.lim = N / 2 + 1;
+    ix.anyq = 0;
 #pragma unroll
     for (int j = 0; j < CH / 2; ++j) {
       const int q = t + (c * (CH / 2) + j) * T;
@@ -402,7 +407,8 @@ __device__ __forceinline__ ChunkIdx<CH> own_chunk(int c, int sym) {
     }
   } else {
     ix.sym = 0;
-    ix.half = 0;
+    ix.lim = 0;
+    ix.anyq = 0;
 #pragma unroll
     for (int i = 0; i < CH; ++i) {
       const int x = t + (c * CH + i) * T;
@@ -419,17 +425,17 @@ __device__ __forceinline__ void load_coef(const rks_tableau& tb, const char* cba
                                           const ChunkIdx<CH>& ix, cx<RS> (&w)[CH]) {
   const char* q = cbase + (size_t)tb.coef_off[c] * M;
   const int kind = tb.coef_rs[c];
-  // Mirror pairs: FP32-state kernels issue one load per pair (the q = 0
-  // pair also loads n/2).  The FP64-state kernels have no registers to spare
+  // Mirror pairs: FP32-state kernels issue one load per pair (plus the n/2
+  // partner of q = 0).  The FP64-state kernels have no registers to spare
   // for that second code path, so their odd slot repeats the even slot's
-  // address (n/2 for q = 0) and the caches serve the repeat.
+  // address and the caches serve the repeat.
   int ci[CH];
 #pragma unroll
   for (int i = 0; i < CH; ++i) ci[i] = ix.m[i];
   if constexpr (CH % 2 == 0 && sizeof(RS) > 4) {
 #pragma unroll
     for (int j = 0; j < CH / 2; ++j)
-      if (ix.sym) ci[2 * j + 1] = (ix.m[2 * j + 1] == ix.half) ? ix.half : ix.m[2 * j];
+      if (ix.sym) ci[2 * j + 1] = (ix.m[2 * j + 1] < ix.lim) ? ix.m[2 * j + 1] : ix.m[2 * j];
   }
   auto gather = [&](auto ld) {
     if constexpr (CH % 2 == 0 && sizeof(RS) == 4) {
@@ -439,7 +445,13 @@ __device__ __forceinline__ void load_coef(const rks_tableau& tb, const char* cba
           w[2 * j] = ld(ix.m[2 * j]);
           w[2 * j + 1] = w[2 * j];
         }
-        if (ix.m[1] == ix.half) w[1] = ld(ix.half);
+        if (ix.anyq) {
+#pragma unroll
+          for (int j = 0; j < CH / 2; ++j)
+            if (ix.m[2 * j + 1] < ix.lim) w[2 * j + 1] = ld(ix.m[2 * j + 1]);
+        } else if (ix.m[1] < ix.lim) {
+          w[1] = ld(ix.m[1]);
+        }
         return;
       }
     }
