@@ -130,7 +130,9 @@ int rks_nl_eval(rks_plan* p, const void* const* u, void* const* nl_out, hipStrea
 
 /* Coefficient arrays of instance `inst` at step h (e.g. _Etd35Diagonal.update_coeffs,
  * etd35.py:157-288), computed in FP64 and stored in the plan precision.
- * dst (optional, device) receives ncoef x m complex values. */
+ * dst (optional, device) receives ncoef x m complex values (every mode, also where
+ * the stepping kernels keep a mirror-symmetric lin_op's modes 0..n/2 only); the
+ * stepping kernels regenerate their own coefficient cache after this call. */
 int rks_coeffs(rks_plan* p, int inst, double h, void* dst, hipStream_t stream);
 /* Generic-nl path: one stage combination k = E u + sum C NL (e.g. etd35.py:320-343),
  * stage = -1 gives the embedded error estimate (etd35.py:344).  slots: nslot device
