@@ -328,13 +328,15 @@ int rks_plan_create(int method, int nl_kind, int precision, const rks_config* cf
   p->esize = precision == RKS_F32 ? 8 : 16;
   p->nsize = precision == RKS_F64 ? 16 : 8;
   // coefficient layout: index order for uniform precision; propagators (state
-  // precision) first, then the rest, for mixed precision.  FP32 and mixed
-  // plans whose lin_op is real everywhere store every coefficient real (z = h L
-  // is real, so are exp and the psi functions; the contour means' imaginary
-  // parts are rounding) at half the bytes; FP64 plans keep the reference's
-  // complex coefficients.
+  // precision) first, then the rest, for mixed precision.  Plans with a
+  // built-in term whose lin_op is real everywhere store every coefficient real
+  // at half the bytes: z = h L is real, so are exp and the psi functions, and
+  // the reference's complex coefficients have imaginary parts that are exactly
+  // zero, or rounding (below 2e-15 of the array's largest value) in the
+  // contour means, whose nodes come in conjugate pairs.  The generic path (no built-in term) keeps the
+  // reference's complex arrays.
   {
-    bool real = precision != RKS_F64 && nl_kind != RKS_NL_NONE;
+    bool real = nl_kind != RKS_NL_NONE;
     for (int i = 0; i < ninst && real; ++i) real = !desc[i].L_is_complex;
     p->coef_real = real ? 1 : 0;
     const int rs_b = (precision == RKS_MIXED ? 16 : p->esize) / (real ? 2 : 1);
@@ -799,10 +801,15 @@ int rks_coeffs(rks_plan* p, int inst, double h, void* dst, hipStream_t stream) {
   hipLaunchKernelGGL(rks_coef_invalidate, dim3(1), dim3(1), 0, stream, p->d_inst + inst);
   HIPCHK(hipGetLastError());
   if (dst && p->coef_real) {
-    // F32 plan with real coefficient storage: ncoef x M floats, returned as complex
+    // real coefficient storage: ncoef x M reals, returned as complex
     const size_t n = (size_t)p->tab.ncoef * M;
-    hipLaunchKernelGGL(rks_real_to_complex<float>, dim3((unsigned)((n + 255) / 256 < 4096 ? (n + 255) / 256 : 4096)),
-                       dim3(256), 0, stream, (const float*)cbase, (cx<float>*)dst, n);
+    const dim3 grid((unsigned)((n + 255) / 256 < 4096 ? (n + 255) / 256 : 4096));
+    if (p->prec == RKS_F64)
+      hipLaunchKernelGGL(rks_real_to_complex<double>, grid, dim3(256), 0, stream, (const double*)cbase,
+                         (cx<double>*)dst, n);
+    else
+      hipLaunchKernelGGL(rks_real_to_complex<float>, grid, dim3(256), 0, stream, (const float*)cbase,
+                         (cx<float>*)dst, n);
     HIPCHK(hipGetLastError());
   } else if (dst) {
     HIPCHK(hipMemcpyAsync(dst, cbase, nbytes, hipMemcpyDeviceToDevice, stream));

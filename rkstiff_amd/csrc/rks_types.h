@@ -26,8 +26,8 @@
 #define RKS_NL1_FSAL 0       // NL1 <- NL(last stage)   (IF34, ETD34, IF45DP, constant methods)
 #define RKS_NL1_RECOMPUTE 1  // NL1 <- nl(u)            (ETD35, etd35.py:317-318)
 
-// coefficient storage kinds (rks_tableau.coef_rs): complex or real (real
-// lin_op in FP32/mixed plans, where every coefficient is real), in NL
+// coefficient storage kinds (rks_tableau.coef_rs): complex or real (a real
+// lin_op with a built-in term, where every coefficient is real), in NL
 // storage precision RN or state precision RS
 #define RKS_CST_RN 0
 #define RKS_CST_RS 1
