@@ -258,6 +258,33 @@ def _nls_drift(n=256, drift=0.4):
     return 1j * (-0.5 * k**2 + drift * k), u0
 
 
+@pytest.mark.parametrize("tile", [0, 1])
+def test_cubic_mirror_pairs_every_size(rk, tile):
+    """Mirror-pair ownership at every cubic size bucket (n = 16: 8 threads per trajectory, up to
+    8192: 512), symmetric and drifting lin_ops in one plan, both FFT tiles where the second
+    exists (n = 512..4096): FP64 engine vs oracle runs."""
+    from rkstiff_amd.batch import BatchSolver
+    sizes = (16, 32, 64, 128, 512, 1024, 2048, 4096, 8192)
+    cases = [(n, d) for n in sizes for d in (0.0, 0.4)]
+    probs, u0s, refs = [], [], []
+    for n, d in cases:
+        L, u0 = _nls_drift(n, d)
+        probs.append((torch.as_tensor(L), rk.CubicNL(n, 1.0)))
+        u0s.append(u0)
+        drv = solvers.make_solver("ETD35", L, problems.nls_nl(1.0), solvers.SolverConfig(epsilon=1e-6))
+        refs.append(drv.evolve(u0, t0=0.0, tf=1.0, store_data=False))
+    bs = BatchSolver("ETD35", probs, precision="f64", config=rk.SolverConfig(epsilon=1e-6))
+    if tile:
+        buckets = bs.plan.buckets()
+        bs.plan.set_launch(True, None, [1 if ntile > 1 else 0 for (_, _, _, ntile) in buckets])
+    uf = bs.evolve([dev(u) for u in u0s], 0.0, 1.0)
+    assert not bs.failures()
+    for (n, d), u, ref in zip(cases, uf, refs):
+        err = rel(host(u), ref)
+        print(f"n={n} drift={d} tile={tile} rel {err:.2e}", flush=True)
+        assert err < 1e-10, (n, d)
+
+
 @pytest.mark.parametrize("method", ADAPTIVE + CONSTANT)
 def test_cubic_engine_asymmetric_L(rk, method):
     """Mirror modes share coefficients only when L[m] == L[n - m]; a drift term breaks that, and
