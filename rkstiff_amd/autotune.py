@@ -14,12 +14,18 @@ per bucket.  Two launch choices change throughput and not results:
   passes and transforms.
 
 ``tune_launch`` times short pilot evolves of the plan's own trajectories and
-keeps the fastest shape.  It tries concurrent (the plan default) and
-sequential buckets first,
-then, bucket by bucket (largest n first), the second tile and then half and
-a quarter of the resident workgroups.  A change is kept only when it is faster by more than
-``min_gain``.  The shape is applied to the plan and returned with the timings
-of every trial.
+keeps the fastest shape.  The per-bucket choices (the second tile, then half
+and a quarter of the resident workgroups) are timed on pilots in which only
+that bucket's trajectories move (the others get tf = t0, which leaves them
+untouched), so one bucket's choice is not lost in the noise of the others;
+a choice that wins there is kept only if the whole ensemble, buckets
+overlapping, is not slower with it.  Then concurrent (the plan default) and
+sequential buckets are compared on the whole ensemble, and the tuned shape
+is kept only if the whole ensemble is faster with it than with the default
+shape.  A change is kept only when
+it is faster by more than ``min_gain``; every timing is the fastest of
+``reps`` pilots after a warm-up pilot.  The shape is applied to the plan and
+returned with the timings of every trial.
 """
 import time
 
@@ -30,19 +36,25 @@ import torch
 def _timed(plan, u0, uf, t0s, tfs, hs, reps):
     plan.evolve(u0, uf, t0s, tfs, hs)  # warm
     torch.cuda.synchronize()
-    t = time.perf_counter()
-    for _ in range(reps):
+    best = float("inf")
+    for _ in range(max(1, reps)):
+        t = time.perf_counter()
         plan.evolve(u0, uf, t0s, tfs, hs)
-    torch.cuda.synchronize()
-    return (time.perf_counter() - t) / reps
+        torch.cuda.synchronize()
+        best = min(best, time.perf_counter() - t)
+    return best
 
 
-def tune_launch(plan, u0, t0, tf, h, reps=1, min_gain=0.02):
+def _log2(n):
+    return int(n).bit_length() - 1
+
+
+def tune_launch(plan, u0, t0, tf, h, reps=2, min_gain=0.02):
     """Tune ``plan``'s launch shape on pilot evolves over [t0, tf] from ``u0``.
 
     u0: list of device state tensors (the plan's dtype), one per trajectory;
     t0/tf/h: scalars or per-trajectory sequences (h: first step).
-    Returns {"concurrent", "wg_per_cu", "seconds", "trials"}.
+    Returns {"concurrent", "wg_per_cu", "tile", "seconds", "trials", "buckets"}.
     """
     n = plan.ninst
     t0s = np.broadcast_to(np.asarray(t0, dtype=np.float64), (n,)).tolist()
@@ -56,40 +68,65 @@ def tune_launch(plan, u0, t0, tf, h, reps=1, min_gain=0.02):
         sec = _timed(plan, u0, uf, t0s, tfs, hs, reps)
         return {"concurrent": None, "wg_per_cu": [], "tile": [], "seconds": sec,
                 "trials": [{"concurrent": None, "wg_per_cu": [], "tile": [], "seconds": sec}], "buckets": []}
+    inst_l2 = [_log2(x) for x in plan.ns]
 
-    def trial(conc, wg, ti):
+    def trial(conc, wg, ti, bucket=None):
         plan.set_launch(conc, wg, ti)
-        sec = _timed(plan, u0, uf, t0s, tfs, hs, reps)
-        trials.append({"concurrent": conc, "wg_per_cu": list(wg), "tile": list(ti), "seconds": sec})
+        if bucket is None:
+            sec = _timed(plan, u0, uf, t0s, tfs, hs, reps)
+        else:
+            l2 = buckets[bucket][0]
+            only = [tfs[i] if inst_l2[i] == l2 else t0s[i] for i in range(n)]
+            sec = _timed(plan, u0, uf, t0s, only, hs, reps)
+        trials.append({"concurrent": conc, "wg_per_cu": list(wg), "tile": list(ti), "bucket": bucket,
+                       "seconds": sec})
         return sec
 
     nb = len(buckets)
-    best_wg, best_ti = [0] * nb, [0] * nb
-    best_conc = True  # the plan's default shape is the first trial
-    best = trial(True, best_wg, best_ti)
+    wg0, ti0 = [0] * nb, [0] * nb
+    default = trial(True, wg0, ti0)  # the plan's default shape
+    best_wg, best_ti, whole = list(wg0), list(ti0), default
+
+    def consider(b, base, wg, ti):
+        """Keep (wg, ti) if bucket b alone gains min_gain and the whole ensemble is not slower."""
+        nonlocal whole
+        sec = trial(True, wg, ti, b if nb > 1 else None)
+        if not sec < base * (1.0 - min_gain):
+            return None
+        if nb > 1:
+            w = trial(True, wg, ti)
+            if w > whole:
+                return None
+            whole = w
+        else:
+            whole = sec
+        return sec
+
+    for b, (_l2, _cnt, _mx, ntile) in enumerate(buckets):
+        # this bucket alone: tile, then the resident-workgroup limit of the chosen tile
+        base = trial(True, best_wg, best_ti, b) if nb > 1 else default
+        if ntile > 1:
+            ti = list(best_ti)
+            ti[b] = 1
+            sec = consider(b, base, best_wg, ti)
+            if sec is not None:
+                base, best_ti = sec, ti
+        plan.set_launch(True, best_wg, best_ti)
+        if best_ti[b] == 0:
+            mx = plan.buckets()[b][2]
+            for v in sorted({mx // 2, mx // 4} - {0, mx}, reverse=True):
+                wg = list(best_wg)
+                wg[b] = v
+                sec = consider(b, base, wg, best_ti)
+                if sec is not None:
+                    base, best_wg = sec, wg
+    best_conc, best = True, whole
     if nb > 1:
         sec = trial(False, best_wg, best_ti)
         if sec < best * (1.0 - min_gain):
             best, best_conc = sec, False
-    for b, (_l2, _cnt, _mx, ntile) in enumerate(buckets):
-        if ntile > 1:
-            ti = list(best_ti)
-            ti[b] = 1
-            sec = trial(best_conc, best_wg, ti)
-            if sec < best * (1.0 - min_gain):
-                best, best_ti = sec, ti
-    plan.set_launch(best_conc, best_wg, best_ti)
-    for b in range(nb):
-        # resident-workgroup limit of the chosen tile
-        mx = plan.buckets()[b][2] if best_ti[b] == 0 else None
-        if mx is None:
-            continue
-        for v in sorted({mx // 2, mx // 4} - {0, mx}, reverse=True):
-            wg = list(best_wg)
-            wg[b] = v
-            sec = trial(best_conc, wg, best_ti)
-            if sec < best * (1.0 - min_gain):
-                best, best_wg = sec, wg
+        if not best < default:
+            best, best_conc, best_wg, best_ti = default, True, list(wg0), list(ti0)
     plan.set_launch(best_conc, best_wg, best_ti)
     return {"concurrent": best_conc, "wg_per_cu": best_wg, "tile": best_ti, "seconds": best, "trials": trials,
             "buckets": [{"log2n": l2, "count": c, "max_wg_per_cu": m, "tiles": t} for l2, c, m, t in buckets]}

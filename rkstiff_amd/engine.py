@@ -65,6 +65,7 @@ class Plan:
         arr = (_lib.InstanceDesc * len(descs))()
         keep = []
         self.modes = []
+        self.ns = []
         for i, d in enumerate(descs):
             L, cplx = lin_op_f64(d["L"])
             Lv = torch.view_as_real(L).reshape(-1) if cplx else L
@@ -82,6 +83,7 @@ class Plan:
             arr[i].ny = int(d.get("ny", 0))
             m = L.numel()
             self.modes.append(m)
+            self.ns.append(int(d["n"]))
         handle = ctypes.c_void_p()
         _lib.check(lib.rks_plan_create(_lib.METHOD_IDS[method], self.nl_kind, precision,
                                        ctypes.byref(self.cfg), len(descs), arr, ctypes.byref(handle)))

@@ -329,7 +329,21 @@ def test_launch_shapes_do_not_change_results():
         for a, b in zip(out, ref):
             assert torch.equal(a, b)
     info = tune_launch(plan, u0, 0.0, 1.0, 0.01)
-    print("tuned", info["concurrent"], info["wg_per_cu"], [round(t["seconds"], 4) for t in info["trials"]])
-    assert info["seconds"] == min(t["seconds"] for t in info["trials"]) or info["seconds"] <= info["trials"][0]["seconds"]
+    print("tuned", info["concurrent"], info["wg_per_cu"], info["tile"],
+          [(t["bucket"], round(t["seconds"], 4)) for t in info["trials"]])
+    # the tuned shape is never slower than the default on the whole ensemble, and per-bucket
+    # trials cover every bucket.  The shape left on the plan keeps the trajectories bit-identical
+    # unless it chose the second FFT tile somewhere (another pass structure: rounding-level
+    # differences, held to the tile parity tests)
+    assert info["seconds"] <= info["trials"][0]["seconds"]
+    assert {t["bucket"] for t in info["trials"]} >= set(range(len(buckets)))
+    _, out, c2 = run()
+    if not any(info["tile"]):
+        assert c2 == cnt
+        for a, b in zip(out, ref):
+            assert torch.equal(a, b)
+    else:
+        for a, b in zip(out, ref):
+            assert float(torch.linalg.norm(a - b) / torch.linalg.norm(b)) < 1e-10
     with pytest.raises(ValueError):
         plan.set_launch(False, [1])
