@@ -71,8 +71,9 @@ struct Geo2 {
   static constexpr int CCH = CEPT < CCH_MAX ? CEPT : CCH_MAX;
   static constexpr int NCG = (NXH + CG - 1) / CG;
   static constexpr int COL_LDS = (CG * PS + TWS) * (int)sizeof(cx<R>);
-  // row kernel: RG complex transforms (2 RG rows), RTG threads each
-  static constexpr int RG = sizeof(R) == 4 ? 8 : 4;
+  // row kernel: RG complex transforms (2 RG rows), RTG threads each (N = 512:
+  // 64 threads, 8 points each, three radix-8 passes)
+  static constexpr int RG = 8;
   static constexpr int RTG = (N / 2 < 512 / RG) ? N / 2 : 512 / RG;
   static constexpr int RT = RG * RTG;
   static constexpr int NRG = N / (2 * RG);
