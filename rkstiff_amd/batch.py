@@ -90,7 +90,7 @@ class BatchSolver:
             self._stats = self.plan.stats()
         return uf
 
-    def autotune(self, u0, t0, tf_probe, h=None, reps=1):
+    def autotune(self, u0, t0, tf_probe, h=None, reps=2):
         """Pick the launch shape (rkstiff_amd.autotune) on pilot evolves t0 -> tf_probe."""
         from .autotune import tune_launch
         u0 = [u.to(self.dtype).contiguous() for u in u0]
