@@ -27,7 +27,7 @@ Also measured and reported under "workloads" (one timed pass each):
   learned_dt    the learned step-size proposer on a mixed batch: KS as above
                 (IF34, epsilon 1e-5, tf 100) and NLS with n ~ U{1024..8192} and
                 1-3 random solitons (ETD35, epsilon 1e-6, tf 20), --learned-count
-                of each, FP32.  The network is trained on the device on ~2e5
+                (default 4096, 8192 trajectories in all) of each, FP32.  The network is trained on the device on ~2e5
                 records from the solver's own runs (separate seeds); the line
                 reports the reference controller and the proposer side by side
                 (trajectories/s, attempts, rejections, coefficient
@@ -399,7 +399,7 @@ def main():
     ap.add_argument("--workloads", default="nls,ks,ac2d,learned")
     ap.add_argument("--ks-count", type=int, default=4096)
     ap.add_argument("--ac-batch", type=int, default=64)
-    ap.add_argument("--learned-count", type=int, default=1024, help="trajectories per family")
+    ap.add_argument("--learned-count", type=int, default=4096, help="trajectories per family")
     ap.add_argument("--learned-records", type=int, default=200000)
     ap.add_argument("--learned-steps", type=int, default=2000)
     ap.add_argument("--copy-mib", type=int, default=0, help="device copy of this size first (counter scale)")
