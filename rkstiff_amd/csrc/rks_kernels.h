@@ -346,8 +346,8 @@ __device__ __attribute__((noinline)) void coef_update(const rks_tableau& tb_in, 
 
 // ---------------------------------------------------------------------------
 // Stage combinations and the error estimate, chunked and term-major: CH modes
-// m = m0 + i*T of one thread at once, so every term issues CH independent
-// loads.  Arithmetic is in the state precision RS; NL arrays and the non-
+// of one thread at once (a ChunkIdx: m0 + i*T, or the persistent kernel's
+// own_chunk map), so every term issues CH independent loads.  Arithmetic is in the state precision RS; NL arrays and the non-
 // propagator coefficients may be stored in RN (converted on load).
 //   term (rks_term grouping):  inner = NL[src] + scale2 * NL[src2]
 //     hmode 0: scale * (C * inner)   1: ((h * C) * inner) * scale   2: ((scale*h)/den) * inner
