@@ -24,6 +24,7 @@
 // others advance.  The host captures the sequence once into a hipGraph and
 // replays it until no trajectory is running.
 #pragma once
+#include "rks_2d_host.h"
 #include "rks_kernels.h"
 
 #define NLK_AC2D 3
@@ -58,6 +59,9 @@ struct Geo2 {
   static constexpr int N = 1 << LOG2;
   static constexpr int NXH = N / 2 + 1;
   static constexpr int M = N * NXH;
+  // row stride of the transform buffer T (elements): whole 128-byte lines, so
+  // a column tile's row segments do not straddle cache lines
+  static constexpr int TXS = rks2_t_stride(N, (int)sizeof(cx<R>));
   static constexpr int PS = rksfft::padded_size(N);
   static constexpr int TWS = rksfft::TwSplit<LOG2>::SIZE;
   // column kernel: CG columns of N points, CTG threads per column
@@ -162,7 +166,7 @@ __global__ __launch_bounds__((Geo2<R, LOG2>::CT), 4) void k2_col(rks2_launch P, 
 #pragma unroll
     for (int i = 0; i < CEPT; ++i) {
       const int r = r0 + i * RS;
-      bc[pad(r)] = valid ? Tb[r * NXH + col] : zero;
+      bc[pad(r)] = valid ? Tb[r * G::TXS + col] : zero;
     }
     __syncthreads();
     rksfft::fft_multi<R, N, G::CTG, CG, false, LOG2, RKS_MAXR>(buf, twl);
@@ -236,7 +240,7 @@ __global__ __launch_bounds__((Geo2<R, LOG2>::CT), 4) void k2_col(rks2_launch P, 
 #pragma unroll
       for (int i = 0; i < CEPT; ++i) {
         const int r = r0 + i * RS;
-        Tb[r * NXH + col] = bc[pad(r)];
+        Tb[r * G::TXS + col] = bc[pad(r)];
       }
     }
   }
@@ -268,8 +272,8 @@ __global__ __launch_bounds__((Geo2<R, LOG2>::RT)) void k2_row(rks2_launch P, int
   const int row0 = blockIdx.x * 2 * RG;
   for (int e = t; e < RG * NXH; e += RT) {
     const int g = e / NXH, k = e - g * NXH;
-    const C a = Tb[(row0 + 2 * g) * NXH + k];
-    const C b = Tb[(row0 + 2 * g + 1) * NXH + k];
+    const C a = Tb[(row0 + 2 * g) * G::TXS + k];
+    const C b = Tb[(row0 + 2 * g + 1) * G::TXS + k];
     C* bg = buf + g * PS;
     // full spectrum of w1 + i w2 from the two half spectra; DC and Nyquist keep
     // only their real parts, as irfft does
@@ -298,8 +302,8 @@ __global__ __launch_bounds__((Geo2<R, LOG2>::RT)) void k2_row(rks2_launch P, int
     const C y = bg[pad((N - k) & (N - 1))];
     // Z = F1 + i F2 with F1, F2 Hermitian: F1 = (Z + conj Y)/2, F2 = (Z - conj Y)/(2i)
     const R hf = (R)0.5;
-    Tb[(row0 + 2 * g) * NXH + k] = mkc((z.re + y.re) * hf, (z.im - y.im) * hf);
-    Tb[(row0 + 2 * g + 1) * NXH + k] = mkc((z.im + y.im) * hf, (y.re - z.re) * hf);
+    Tb[(row0 + 2 * g) * G::TXS + k] = mkc((z.re + y.re) * hf, (z.im - y.im) * hf);
+    Tb[(row0 + 2 * g + 1) * G::TXS + k] = mkc((z.im + y.im) * hf, (y.re - z.re) * hf);
   }
 }
 

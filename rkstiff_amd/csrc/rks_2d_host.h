@@ -13,11 +13,17 @@ struct rks2_ctx;
 int rks2_supported(int prec, int log2n);
 // modes of an N x N rfft2 grid
 inline int rks2_modes(int log2n) { return (1 << log2n) * ((1 << log2n) / 2 + 1); }
+// row stride (elements) of the transform buffer T: N/2 + 1 columns rounded up
+// to whole 128-byte lines of esize-byte elements
+__host__ __device__ constexpr int rks2_t_stride(int n, int esize) {
+  return ((n / 2 + 1) * esize + 127) / 128 * 128 / esize;
+}
 
 // Allocates the per-trajectory flags and reduction partials and sets kernel
 // attributes.  The workspace of instance i holds (byte offsets)
 //   NL slots at 0 | cand0 at off_cand0 | cand1 at off_cand1 | coefficients at
-//   off_coef | transform buffer T (M state-precision complex) at off_T.
+//   off_coef | transform buffer T (N rows of rks2_t_stride state-precision
+//   complex values) at off_T.
 int rks2_create(rks2_ctx** out, int prec, int log2n, int ninst, const rks_tableau* d_tab, const rks_ctrl* d_ctl,
                 rks_inst* d_inst, const rks_io* d_io, const void* d_tw, size_t off_cand0, size_t off_cand1,
                 size_t off_coef, size_t off_T);

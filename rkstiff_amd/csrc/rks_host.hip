@@ -416,7 +416,8 @@ int rks_plan_create(int method, int nl_kind, int precision, const rks_config* cf
     p->ws_off[i] = ws_total;
     ws_total += rks_align256((size_t)p->tab.nslot * m * p->nsize) + 2 * rks_align256((size_t)m * p->esize) +
                 rks_align256(p->coef_bytes_per_mode * m);
-    if (is2d) ws_total += rks_align256((size_t)m * p->esize);  // transform buffer T
+    if (is2d)  // transform buffer T, rows padded to whole cache lines
+      ws_total += rks_align256((size_t)n * rks2_t_stride(n, (int)p->esize) * p->esize);
     Loff[i] = L_total;
     L_total += (size_t)m * (desc[i].L_is_complex ? 2 : 1);
     kxoff[i] = kx_total;
