@@ -148,7 +148,7 @@ def bytes_per_accept(m, e_u, e_nl):
 # HBM bytes per trajectory of the headline workload measured with rocprofv3
 # FETCH_SIZE / WRITE_SIZE (scaled by a copy of known size;
 # profiles/rks_kernel_nls8192_etd35_b256_hbm_counters.txt)
-TRAFFIC_PER_TRAJECTORY = {"f64": 9.03e9, "mixed": 6.22e9, "f32": 4.03e9}
+TRAFFIC_PER_TRAJECTORY = {"f64": 9.04e9, "mixed": 6.23e9, "f32": 4.03e9}
 
 MODES = {  # name: (precision id, state element bytes, NL element bytes)
     "f32": ("F32", 8, 8),
